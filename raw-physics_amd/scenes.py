@@ -1,0 +1,373 @@
+"""Seeded synthetic scenes for the raw-physics step, as flat world descriptors.
+
+A world is a batch of independent scenes. Each body references a *shape*: a
+triangle corner stream of float32 xyz (three corners per triangle), exactly
+what the reference's obj_parse returns (src/render/obj.cpp:45-107). The hull
+builder widens it to double and multiplies by the body's scale, as
+examples_util_create_single_convex_hull_collider_array does
+(src/examples/examples_util.cpp:5-24).
+
+Workloads (sizes as named in the benchmark configuration):
+  * ``stack_scene``      — 10 unit cubes stacked on a 50x1x50 static floor.
+  * ``batched_uniform``  — N scenes of a static floor + 8 cubes + 8 42-vertex
+                           icospheres dropped from a 4x4 grid.
+
+Random numbers: SplitMix64 seeds a xoshiro256** stream per scene with seed
+0x5EED0000 + scene index; u = (x >> 11) * 2^-53.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# --------------------------------------------------------------------------
+# Shapes
+# --------------------------------------------------------------------------
+# res/cube.obj: 8 vertices and 12 triangles (face lines, 1-based). The corner
+# order fixes the hull's vertex and face order, which the step's results
+# depend on (first-max tie-breaks), so the asset is reproduced as data.
+_CUBE_V = [(1.0, 1.0, -1.0), (1.0, -1.0, -1.0), (1.0, 1.0, 1.0), (1.0, -1.0, 1.0),
+           (-1.0, 1.0, -1.0), (-1.0, -1.0, -1.0), (-1.0, 1.0, 1.0), (-1.0, -1.0, 1.0)]
+_CUBE_F = [(5, 3, 1), (3, 8, 4), (7, 6, 8), (2, 8, 6), (1, 4, 2), (5, 2, 6),
+           (5, 7, 3), (3, 7, 8), (7, 5, 6), (2, 4, 8), (1, 3, 4), (5, 1, 2)]
+
+# res/ico.obj: the 12-vertex icosahedron (vertex values as written in the file).
+_ICO_V = [(0.000000, -1.000000, 0.000000), (0.723600, -0.447215, 0.525720),
+          (-0.276385, -0.447215, 0.850640), (-0.894425, -0.447215, 0.000000),
+          (-0.276385, -0.447215, -0.850640), (0.723600, -0.447215, -0.525720),
+          (0.276385, 0.447215, 0.850640), (-0.723600, 0.447215, 0.525720),
+          (-0.723600, 0.447215, -0.525720), (0.276385, 0.447215, -0.850640),
+          (0.894425, 0.447215, 0.000000), (0.000000, 1.000000, 0.000000)]
+_ICO_F = [(1, 2, 3), (2, 1, 6), (1, 3, 4), (1, 4, 5), (1, 5, 6), (2, 6, 11), (3, 2, 7), (4, 3, 8),
+          (5, 4, 9), (6, 5, 10), (2, 11, 7), (3, 7, 8), (4, 8, 9), (5, 9, 10), (6, 10, 11),
+          (7, 11, 12), (8, 7, 12), (9, 8, 12), (10, 9, 12), (11, 10, 12)]
+
+
+def _corners(verts, faces) -> np.ndarray:
+    out = np.empty((len(faces) * 3, 3), dtype=np.float32)
+    for t, f in enumerate(faces):
+        for k in range(3):
+            out[3 * t + k] = verts[f[k] - 1]
+    # no negative zeros: the reference's vertex hash keys on the bit pattern
+    out += np.float32(0.0)
+    return out
+
+
+def cube_corners() -> np.ndarray:
+    """Corner stream of res/cube.obj (36 corners, 8 unique vertices)."""
+    return _corners(_CUBE_V, _CUBE_F)
+
+
+def icosphere_corners(subdivisions: int) -> np.ndarray:
+    """Unit-radius icosphere: res/ico.obj's icosahedron with every triangle
+    split at its edge midpoints `subdivisions` times, vertices projected to
+    the unit sphere (12 / 42 / 162 vertices for 0 / 1 / 2 subdivisions)."""
+    verts = [tuple(float(np.float32(c)) for c in v) for v in _ICO_V]
+    if subdivisions > 0:
+        verts = [tuple(c / math.sqrt(sum(x * x for x in v)) for c in v) for v in verts]
+    faces = [tuple(f) for f in _ICO_F]
+    for _ in range(subdivisions):
+        mid = {}
+
+        def midpoint(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in mid:
+                va, vb = verts[a - 1], verts[b - 1]
+                m = [(va[i] + vb[i]) * 0.5 for i in range(3)]
+                n = math.sqrt(sum(x * x for x in m))
+                verts.append(tuple(x / n for x in m))
+                mid[key] = len(verts)
+            return mid[key]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = midpoint(a, b), midpoint(b, c), midpoint(c, a)
+            nf += [(a, ab, ca), (ab, b, bc), (ca, bc, c), (ab, bc, ca)]
+        faces = nf
+    return _corners(verts, faces)
+
+
+def hull_volume(corners: np.ndarray, scale) -> float:
+    """Volume of the closed triangle mesh (divergence theorem), scaled."""
+    p = corners.astype(np.float64) * np.asarray(scale, dtype=np.float64)
+    a, b, c = p[0::3], p[1::3], p[2::3]
+    return float(abs(np.einsum("ij,ij->i", a, np.cross(b, c)).sum()) / 6.0)
+
+
+# --------------------------------------------------------------------------
+# Random numbers
+# --------------------------------------------------------------------------
+_M64 = (1 << 64) - 1
+
+
+class Rng:
+    """xoshiro256** seeded by SplitMix64."""
+
+    def __init__(self, seed: int):
+        s = seed & _M64
+        self.s = []
+        for _ in range(4):
+            s = (s + 0x9E3779B97F4A7C15) & _M64
+            z = s
+            z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+            z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+            self.s.append(z ^ (z >> 31))
+
+    def next(self) -> int:
+        s = self.s
+        r = (((s[1] * 5) & _M64) << 7 | ((s[1] * 5) & _M64) >> 57) & _M64
+        r = (r * 9) & _M64
+        t = (s[1] << 17) & _M64
+        s[2] ^= s[0]
+        s[3] ^= s[1]
+        s[1] ^= s[2]
+        s[0] ^= s[3]
+        s[2] ^= t
+        s[3] = ((s[3] << 45) | (s[3] >> 19)) & _M64
+        return r
+
+    def uniform(self, a: float = 0.0, b: float = 1.0) -> float:
+        return a + (b - a) * ((self.next() >> 11) * (1.0 / 9007199254740992.0))
+
+    def normal(self, sigma: float = 1.0) -> float:
+        u1 = max(self.uniform(), 1e-300)
+        u2 = self.uniform()
+        return sigma * math.sqrt(-2.0 * math.log(u1)) * math.cos(2.0 * math.pi * u2)
+
+    def quaternion(self):
+        """Uniform random rotation (Shoemake), as x, y, z, w."""
+        u1, u2, u3 = self.uniform(), self.uniform(), self.uniform()
+        a, b = math.sqrt(1.0 - u1), math.sqrt(u1)
+        return (a * math.sin(2 * math.pi * u2), a * math.cos(2 * math.pi * u2),
+                b * math.sin(2 * math.pi * u3), b * math.cos(2 * math.pi * u3))
+
+
+PI_F = 3.14159265358979  # include/gm.h:8
+
+
+def quaternion_new(axis, degrees):
+    """quaternion_new (src/quaternion.cpp:18-31): axis-angle in degrees."""
+    ln = math.sqrt(axis[0] * axis[0] + axis[1] * axis[1] + axis[2] * axis[2])
+    if ln != 0.0:
+        axis = (axis[0] / ln, axis[1] / ln, axis[2] / ln)
+    rad = PI_F * degrees / 180.0
+    s = math.sin(rad / 2.0)
+    return (axis[0] * s, axis[1] * s, axis[2] * s, math.cos(rad / 2.0))
+
+
+# --------------------------------------------------------------------------
+# World descriptor
+# --------------------------------------------------------------------------
+# Constraint_Type / PBD_Axis_Type numeric values (src/physics/pbd.h:5-20)
+POSITIONAL, COLLISION, MUTUAL_ORIENTATION, HINGE, SPHERICAL = range(5)
+POS_X, NEG_X, POS_Y, NEG_Y, POS_Z, NEG_Z = range(6)
+
+
+class ConstraintDesc(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("body1", ctypes.c_uint32), ("body2", ctypes.c_uint32),
+                ("axis", ctypes.c_int32 * 4), ("limited", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("r1_lc", ctypes.c_double * 3), ("r2_lc", ctypes.c_double * 3),
+                ("compliance", ctypes.c_double), ("distance", ctypes.c_double * 3),
+                ("limits", ctypes.c_double * 4)]
+
+
+_P = ctypes.c_void_p
+
+
+class WorldDescC(ctypes.Structure):
+    """Layout of orc_world_desc / rp_world_desc."""
+    _fields_ = [("num_shapes", ctypes.c_uint32), ("shape_corner_offset", _P), ("shape_corners", _P),
+                ("num_scenes", ctypes.c_uint32), ("scene_body_offset", _P), ("body_shape", _P),
+                ("body_scale", _P), ("body_position", _P), ("body_rotation", _P),
+                ("body_linear_velocity", _P), ("body_angular_velocity", _P), ("body_mass", _P),
+                ("body_fixed", _P), ("body_material", _P), ("scene_constraint_offset", _P),
+                ("constraints", _P)]
+
+
+class StateC(ctypes.Structure):
+    """Layout of orc_state / rp_state: ten optional output arrays."""
+    _fields_ = [(n, _P) for n in ("position", "rotation", "linear_velocity", "angular_velocity",
+                                  "previous_position", "previous_rotation", "previous_linear_velocity",
+                                  "previous_angular_velocity", "active", "deactivation_time")]
+
+
+STATE_FIELDS = (("position", 3), ("rotation", 4), ("linear_velocity", 3), ("angular_velocity", 3),
+                ("previous_position", 3), ("previous_rotation", 4), ("previous_linear_velocity", 3),
+                ("previous_angular_velocity", 3), ("active", 1), ("deactivation_time", 1))
+
+
+def alloc_state(nb: int) -> dict:
+    st = {n: np.zeros((nb, k), dtype=np.float64) for n, k in STATE_FIELDS if n != "active"}
+    st["active"] = np.zeros((nb, 1), dtype=np.int32)
+    return st
+
+
+def state_struct(st: dict) -> StateC:
+    return StateC(*[st[n].ctypes.data if st.get(n) is not None else None for n, _ in STATE_FIELDS])
+
+
+@dataclass
+class World:
+    """A batch of scenes as flat arrays (the world descriptor's contents)."""
+    shapes: list = field(default_factory=list)           # list of (n,3) float32 corner streams
+    scene_body_offset: list = field(default_factory=lambda: [0])
+    body_shape: list = field(default_factory=list)
+    body_scale: list = field(default_factory=list)
+    body_position: list = field(default_factory=list)
+    body_rotation: list = field(default_factory=list)
+    body_linear_velocity: list = field(default_factory=list)
+    body_angular_velocity: list = field(default_factory=list)
+    body_mass: list = field(default_factory=list)
+    body_fixed: list = field(default_factory=list)
+    body_material: list = field(default_factory=list)
+    scene_constraint_offset: list = field(default_factory=lambda: [0])
+    constraints: list = field(default_factory=list)      # ConstraintDesc
+    name: str = ""
+
+    # -- building -----------------------------------------------------------
+    def shape(self, corners: np.ndarray) -> int:
+        for i, s in enumerate(self.shapes):
+            if s is corners or (s.shape == corners.shape and np.array_equal(s, corners)):
+                return i
+        self.shapes.append(np.ascontiguousarray(corners, dtype=np.float32))
+        return len(self.shapes) - 1
+
+    def add_body(self, shape: int, scale, position, rotation, mass=1.0, fixed=False,
+                 material=(0.5, 0.3, 0.0), linear_velocity=(0.0, 0.0, 0.0),
+                 angular_velocity=(0.0, 0.0, 0.0)) -> int:
+        self.body_shape.append(shape)
+        self.body_scale.append(tuple(scale))
+        self.body_position.append(tuple(position))
+        self.body_rotation.append(tuple(rotation))
+        self.body_linear_velocity.append(tuple(linear_velocity))
+        self.body_angular_velocity.append(tuple(angular_velocity))
+        self.body_mass.append(float(mass))
+        self.body_fixed.append(1 if fixed else 0)
+        self.body_material.append(tuple(material))
+        return len(self.body_shape) - 1 - self.scene_body_offset[-1]
+
+    def add_constraint(self, c: ConstraintDesc):
+        self.constraints.append(c)
+
+    def end_scene(self):
+        self.scene_body_offset.append(len(self.body_shape))
+        self.scene_constraint_offset.append(len(self.constraints))
+
+    @property
+    def num_scenes(self) -> int:
+        return len(self.scene_body_offset) - 1
+
+    @property
+    def num_bodies(self) -> int:
+        return len(self.body_shape)
+
+    def dynamic_bodies(self) -> int:
+        return int(self.num_bodies - sum(self.body_fixed))
+
+    # -- flattening ---------------------------------------------------------
+    def arrays(self) -> dict:
+        offs = [0]
+        for s in self.shapes:
+            offs.append(offs[-1] + s.shape[0])
+        a = {
+            "shape_corner_offset": np.asarray(offs, dtype=np.uint32),
+            "shape_corners": np.ascontiguousarray(np.concatenate(self.shapes, axis=0).reshape(-1), dtype=np.float32),
+            "scene_body_offset": np.asarray(self.scene_body_offset, dtype=np.uint32),
+            "body_shape": np.asarray(self.body_shape, dtype=np.uint32),
+            "body_scale": np.asarray(self.body_scale, dtype=np.float64).reshape(-1),
+            "body_position": np.asarray(self.body_position, dtype=np.float64).reshape(-1),
+            "body_rotation": np.asarray(self.body_rotation, dtype=np.float64).reshape(-1),
+            "body_linear_velocity": np.asarray(self.body_linear_velocity, dtype=np.float64).reshape(-1),
+            "body_angular_velocity": np.asarray(self.body_angular_velocity, dtype=np.float64).reshape(-1),
+            "body_mass": np.asarray(self.body_mass, dtype=np.float64),
+            "body_fixed": np.asarray(self.body_fixed, dtype=np.int32),
+            "body_material": np.asarray(self.body_material, dtype=np.float64).reshape(-1),
+            "scene_constraint_offset": np.asarray(self.scene_constraint_offset, dtype=np.uint32),
+        }
+        if self.constraints:
+            a["constraints"] = (ConstraintDesc * len(self.constraints))(*self.constraints)
+        return a
+
+    def desc(self):
+        """Returns (WorldDescC, keepalive) for the C ABI."""
+        a = self.arrays()
+        cons = a.get("constraints")
+        d = WorldDescC(
+            len(self.shapes), a["shape_corner_offset"].ctypes.data, a["shape_corners"].ctypes.data,
+            self.num_scenes, a["scene_body_offset"].ctypes.data, a["body_shape"].ctypes.data,
+            a["body_scale"].ctypes.data, a["body_position"].ctypes.data, a["body_rotation"].ctypes.data,
+            a["body_linear_velocity"].ctypes.data, a["body_angular_velocity"].ctypes.data,
+            a["body_mass"].ctypes.data, a["body_fixed"].ctypes.data, a["body_material"].ctypes.data,
+            a["scene_constraint_offset"].ctypes.data if cons is not None else None,
+            ctypes.addressof(cons) if cons is not None else None)
+        return d, a
+
+
+# --------------------------------------------------------------------------
+# Workloads
+# --------------------------------------------------------------------------
+def _floor(w: World, material=(0.5, 0.3, 0.0)):
+    """50x1x50 static box at (0,-2,0), top face at y=-1 (src/examples/stack.cpp:48-51)."""
+    cube = w.shape(cube_corners())
+    w.add_body(cube, (50.0, 1.0, 50.0), (0.0, -2.0, 0.0), quaternion_new((0.0, 1.0, 0.0), 0.0), fixed=True,
+               material=material)
+
+
+def stack_scene(w: World | None = None, seed: int = 0, num_cubes: int = 10) -> World:
+    """Plain-method workload: `num_cubes` unit cubes (8-vertex hull, half-extent
+    1, mass 1) stacked on the static floor; jitter +-0.01 on x/z and +-2 deg
+    yaw; friction 0.5/0.3, restitution 0."""
+    w = w or World(name="stack")
+    rng = Rng(0x5EED0000 + seed)
+    _floor(w)
+    cube = w.shape(cube_corners())
+    for k in range(num_cubes):
+        x = rng.uniform(-0.01, 0.01)
+        z = rng.uniform(-0.01, 0.01)
+        yaw = rng.uniform(-2.0, 2.0)
+        w.add_body(cube, (1.0, 1.0, 1.0), (x, 2.01 * k, z), quaternion_new((0.0, 1.0, 0.0), yaw), mass=1.0,
+                   material=(0.5, 0.3, 0.0))
+    w.end_scene()
+    return w
+
+
+GRID_SPACING = 3.6  # > 2 * sqrt(3) * 1.0 + 0.1: no two bounding spheres overlap at t=0
+
+
+def drop_scene(w: World, seed: int, n_cubes: int = 8, n_icos: int = 8, ico_subdiv: int = 1,
+               grid: int = 4) -> World:
+    """One batched-uniform scene: static floor + `n_cubes` cubes (half-extent r)
+    + `n_icos` icospheres (radius r), r ~ U(0.5, 1.0), mass = hull volume
+    (density 1), dropped from a grid x grid lattice at heights U(1, 6) above
+    the floor top, uniformly random orientation, v ~ N(0, 0.5)^3, w = 0."""
+    rng = Rng(0x5EED0000 + seed)
+    _floor(w)
+    cube = w.shape(cube_corners())
+    ico = w.shape(icosphere_corners(ico_subdiv))
+    n = n_cubes + n_icos
+    for k in range(n):
+        is_cube = k < n_cubes
+        r = rng.uniform(0.5, 1.0)
+        gx, gz = k % grid, (k // grid) % grid
+        layer = k // (grid * grid)
+        x = (gx - (grid - 1) / 2.0) * GRID_SPACING
+        z = (gz - (grid - 1) / 2.0) * GRID_SPACING
+        y = -1.0 + rng.uniform(1.0, 6.0) + 7.0 * layer
+        q = rng.quaternion()
+        v = (rng.normal(0.5), rng.normal(0.5), rng.normal(0.5))
+        shape = cube if is_cube else ico
+        scale = (r, r, r)
+        mass = hull_volume(w.shapes[shape], scale)
+        w.add_body(shape, scale, (x, y, z), q, mass=mass, material=(0.5, 0.3, 0.0), linear_velocity=v)
+    w.end_scene()
+    return w
+
+
+def batched_uniform(num_scenes: int, first_seed: int = 0, **kw) -> World:
+    w = World(name=f"batched_uniform_{num_scenes}")
+    for s in range(num_scenes):
+        drop_scene(w, first_seed + s, **kw)
+    return w
