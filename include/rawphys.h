@@ -1,0 +1,286 @@
+/*
+ * rawphys.h — C ABI of the MI355X-native raw-physics step (librawphys.so).
+ *
+ * Two ways in:
+ *
+ *  1. Drop-in for the reference's step (the "plain method"). The library
+ *     exports, with the reference's C++ linkage and signatures
+ *     (src/physics/pbd.h:93-94):
+ *         void pbd_simulate(r64 dt, Entity** entities, u32 num_substeps,
+ *                           u32 num_pos_iters, boolean enable_collisions);
+ *         void pbd_simulate_with_constraints(r64 dt, Entity** entities,
+ *                           Constraint* external_constraints, u32 num_substeps,
+ *                           u32 num_pos_iters, boolean enable_collisions);
+ *     so a program built against the reference headers links against this
+ *     library instead of src/physics/pbd.cpp. The same entry points are
+ *     exported with C linkage below (rp_pbd_simulate, physics_simulate) for
+ *     FFI callers. Entities are read and written in place with the
+ *     reference's struct layout (rp_entity mirrors Entity, src/entity.h:17-47).
+ *
+ *  2. Batched worlds (rp_world_*): thousands of independent scenes resident in
+ *     HBM, stepped frame after frame on the GPU without host round trips.
+ *
+ * All functions are synchronous with respect to the host unless stated.
+ * Errors that the reference turns into assert() aborts are reported as bits
+ * of a per-scene error word (rp_world_error, rp_last_error).
+ */
+#ifndef RAWPHYS_H
+#define RAWPHYS_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- Reference-layout mirrors (include/gm.h:10-134, src/entity.h,
+ *      src/physics/collider.h, src/physics/pbd.h) ------------------------- */
+#pragma pack(push, 1)
+typedef struct { double x, y, z; } rp_vec3;          /* vec3, packed */
+typedef struct { double x, y, z, w; } rp_vec4;       /* vec4, packed */
+typedef struct { double data[3][3]; } rp_mat3;       /* mat3, packed, row-major */
+#pragma pack(pop)
+typedef struct { double x, y, z, w; } rp_quaternion; /* Quaternion (src/quaternion.h:5-7) */
+typedef struct { uint32_t VAO, VBO, EBO, num_indices; } rp_mesh; /* Mesh (src/render/mesh.h:13-16) */
+
+/* Physics_Force (src/entity.h:11-15), 52 bytes */
+typedef struct { rp_vec3 position; rp_vec3 force; int32_t local_coords; } rp_physics_force;
+
+/* Collider_Convex_Hull_Face / Collider_Convex_Hull / Collider_Sphere / Collider
+ * (src/physics/collider.h:12-44). Arrays are light_arrays: the element count
+ * is ((size_t*)p)[-1] (include/light_array.h:73-89). */
+typedef struct { uint32_t* elements; rp_vec3 normal; } rp_hull_face;
+typedef struct {
+    rp_vec3* vertices;
+    rp_vec3* transformed_vertices;
+    rp_hull_face* faces;
+    rp_hull_face* transformed_faces;
+    uint32_t** vertex_to_faces;
+    uint32_t** vertex_to_neighbors;
+    uint32_t** face_to_neighbors;
+} rp_convex_hull;
+typedef struct { float radius; rp_vec3 center; } rp_sphere;
+enum { RP_COLLIDER_SPHERE = 0, RP_COLLIDER_CONVEX_HULL = 1 };
+typedef struct {
+    int32_t type;
+    union { rp_convex_hull convex_hull; rp_sphere sphere; } u;
+} rp_collider;
+
+/* Entity (src/entity.h:17-47), 504 bytes */
+typedef struct {
+    uint64_t id;
+    rp_mesh mesh;
+    rp_vec3 world_position;
+    rp_quaternion world_rotation;
+    rp_vec3 world_scale;
+    rp_vec4 color;
+    rp_collider* colliders;
+    double bounding_sphere_radius;
+    rp_physics_force* forces;
+    double inverse_mass;
+    rp_mat3 inertia_tensor;
+    rp_mat3 inverse_inertia_tensor;
+    rp_vec3 angular_velocity;
+    rp_vec3 linear_velocity;
+    int32_t fixed;
+    int32_t active;
+    double deactivation_time;
+    double static_friction_coefficient;
+    double dynamic_friction_coefficient;
+    double restitution_coefficient;
+    rp_vec3 previous_world_position;
+    rp_quaternion previous_world_rotation;
+    rp_vec3 previous_linear_velocity;
+    rp_vec3 previous_angular_velocity;
+} rp_entity;
+
+/* Constraint (src/physics/pbd.h:5-91), 144 bytes */
+enum {
+    RP_POSITIONAL_CONSTRAINT = 0,
+    RP_COLLISION_CONSTRAINT = 1,
+    RP_MUTUAL_ORIENTATION_CONSTRAINT = 2,
+    RP_HINGE_JOINT_CONSTRAINT = 3,
+    RP_SPHERICAL_JOINT_CONSTRAINT = 4
+};
+typedef struct { rp_vec3 r1_lc, r2_lc; double compliance, lambda; rp_vec3 distance; } rp_positional_constraint;
+typedef struct { rp_vec3 r1_lc, r2_lc, normal; double lambda_t, lambda_n; } rp_collision_constraint;
+typedef struct { double compliance, lambda; } rp_mutual_orientation_constraint;
+typedef struct {
+    rp_vec3 r1_lc, r2_lc;
+    double compliance, lambda_pos;
+    int32_t e1_aligned_axis, e2_aligned_axis;
+    double lambda_aligned_axes;
+    int32_t limited;
+    double upper_limit, lower_limit;
+    int32_t e1_limit_axis, e2_limit_axis;
+    double lambda_limit_axes;
+} rp_hinge_joint_constraint;
+typedef struct {
+    rp_vec3 r1_lc, r2_lc;
+    double lambda_pos, lambda_swing, swing_upper_limit, swing_lower_limit;
+    int32_t e1_swing_axis, e2_swing_axis;
+    double lambda_twist, twist_upper_limit, twist_lower_limit;
+    int32_t e1_twist_axis, e2_twist_axis;
+} rp_spherical_joint_constraint;
+typedef struct {
+    int32_t type;
+    uint64_t e1_id, e2_id;
+    union {
+        rp_positional_constraint positional_constraint;
+        rp_collision_constraint collision_constraint;
+        rp_mutual_orientation_constraint mutual_orientation_constraint;
+        rp_hinge_joint_constraint hinge_joint_constraint;
+        rp_spherical_joint_constraint spherical_joint_constraint;
+    } u;
+} rp_constraint;
+
+/* ---- Drop-in step (plain method) -------------------------------------- */
+
+/* pbd_simulate_with_constraints (src/physics/pbd.h:94, pbd.cpp:468-747) with C
+ * linkage. `entities` is the reference's light_array of Entity*, and
+ * `external_constraints` a light_array of Constraint or NULL. Entities are
+ * updated in place. Returns the error word (0 on success): bits of
+ * RP_ERR_* where the reference would abort. dt <= 0 is a no-op. */
+uint32_t rp_pbd_simulate(double dt, rp_entity** entities, rp_constraint* external_constraints,
+                         uint32_t num_substeps, uint32_t num_pos_iters, int32_t enable_collisions);
+
+/* physics_simulate(entities, dt): pbd_simulate(dt, entities, 20, 1, true), the
+ * reference's default substep configuration (src/examples/stack.cpp:98). */
+void physics_simulate(rp_entity** entities, double dt);
+
+/* ---- Batched worlds ---------------------------------------------------- */
+typedef struct {
+    int32_t type;               /* RP_*_CONSTRAINT except COLLISION */
+    uint32_t body1, body2;      /* scene-local body indices */
+    int32_t axis[4];            /* hinge: e1/e2 aligned, e1/e2 limit; spherical: e1/e2 swing, e1/e2 twist (PBD_Axis_Type) */
+    int32_t limited;            /* hinge */
+    int32_t pad;
+    double r1_lc[3], r2_lc[3];
+    double compliance;          /* positional, mutual orientation, hinge */
+    double distance[3];         /* positional */
+    double limits[4];           /* hinge: lower, upper; spherical: swing lower, swing upper, twist lower, twist upper */
+} rp_constraint_desc;
+
+/* A batch of independent scenes. Shapes are triangle corner streams as the
+ * reference's obj_parse returns them (float xyz per corner, 3 corners per
+ * triangle); a body's convex hull is built from its shape widened to double
+ * and multiplied by the body's scale (src/examples/examples_util.cpp:5-24),
+ * with the reference's hull construction (src/physics/collider.cpp:194-364). */
+typedef struct {
+    uint32_t num_shapes;
+    const uint32_t* shape_corner_offset;  /* [num_shapes+1] */
+    const float* shape_corners;           /* [3 * corners] */
+    uint32_t num_scenes;
+    const uint32_t* scene_body_offset;    /* [num_scenes+1] */
+    const uint32_t* body_shape;           /* [nb] */
+    const double* body_scale;             /* [3nb] */
+    const double* body_position;          /* [3nb] */
+    const double* body_rotation;          /* [4nb] x,y,z,w */
+    const double* body_linear_velocity;   /* [3nb] */
+    const double* body_angular_velocity;  /* [3nb] */
+    const double* body_mass;              /* [nb], ignored for fixed bodies */
+    const int32_t* body_fixed;            /* [nb] */
+    const double* body_material;          /* [3nb] static friction, dynamic friction, restitution */
+    const uint32_t* scene_constraint_offset; /* [num_scenes+1] or NULL */
+    const rp_constraint_desc* constraints;   /* scene-grouped external constraints or NULL */
+} rp_world_desc;
+
+/* Per-body state as flat arrays; NULL members are skipped. */
+typedef struct {
+    double* position;                  /* [3nb] */
+    double* rotation;                  /* [4nb] */
+    double* linear_velocity;           /* [3nb] */
+    double* angular_velocity;          /* [3nb] */
+    double* previous_position;         /* [3nb] */
+    double* previous_rotation;         /* [4nb] */
+    double* previous_linear_velocity;  /* [3nb] */
+    double* previous_angular_velocity; /* [3nb] */
+    int32_t* active;                   /* [nb] */
+    double* deactivation_time;         /* [nb] */
+} rp_state;
+
+typedef struct rp_world rp_world;
+
+/* Error bits (per scene). */
+enum {
+    RP_ERR_GJK_SIMPLEX = 1u << 0,     /* add_to_simplex with 4 points (gjk.cpp:24-26) */
+    RP_ERR_EPA_DEGENERATE = 1u << 1,  /* zero EPA face normal / flat polytope (epa.cpp:41,72) */
+    RP_ERR_SUPPORT_NAN = 1u << 2,     /* no support vertex selected (support.cpp:6-16) */
+    RP_ERR_SKEW_LINES = 1u << 3,      /* parallel edges in an edge contact (clipping.cpp:279) */
+    RP_ERR_ACTIVITY = 1u << 4,        /* colliding non-fixed bodies of different activity (pbd.cpp:589-591) */
+    RP_ERR_HULL = 1u << 5,            /* hull construction / inertia inversion failed */
+    RP_ERR_ZERO_MASS = 1u << 6,       /* w1 + w2 == 0 (pbd_base_constraints.cpp:70,184) */
+    RP_ERR_SELECT = 1u << 7,          /* face / edge / axis selection failed */
+    RP_ERR_EPA_CAPACITY = 1u << 8,    /* EPA polytope exceeded its fixed capacity */
+    RP_ERR_CLIP_CAPACITY = 1u << 9,   /* clipped polygon exceeded its fixed capacity */
+    RP_ERR_CONTACT_CAPACITY = 1u << 10, /* more than 16 contacts for one body pair */
+    RP_ERR_SOLVER_CAPACITY = 1u << 11,  /* constraint list exceeded its capacity */
+    RP_ERR_RUNTIME = 1u << 31         /* HIP runtime failure (see rp_last_error_string) */
+};
+
+/* Creates a world on the current HIP device: builds hulls and derived body
+ * data (entity.cpp:24-65) on the host and uploads everything to HBM.
+ * Returns NULL on failure (rp_last_error_string says why). */
+rp_world* rp_world_create(const rp_world_desc* desc);
+void rp_world_destroy(rp_world* w);
+uint32_t rp_world_num_bodies(const rp_world* w);
+uint32_t rp_world_num_scenes(const rp_world* w);
+
+/* Runs `frames` frames of the reference's caller pattern on every scene
+ * (src/examples/stack.cpp:86-104): each frame a force (0, -gravity*1.0/m^-1, 0)
+ * at every body's origin, then pbd_simulate_with_constraints(dt, scene,
+ * constraints, num_substeps, num_pos_iters, enable_collisions).
+ * Returns 0 or RP_ERR_RUNTIME. Blocks until the GPU is done. */
+uint32_t rp_world_step(rp_world* w, double dt, uint32_t frames, uint32_t num_substeps, uint32_t num_pos_iters,
+                       int32_t enable_collisions, double gravity);
+/* Same, without waiting for completion (call rp_world_synchronize). */
+uint32_t rp_world_step_async(rp_world* w, double dt, uint32_t frames, uint32_t num_substeps, uint32_t num_pos_iters,
+                             int32_t enable_collisions, double gravity);
+uint32_t rp_world_synchronize(rp_world* w);
+
+void rp_world_get_state(rp_world* w, const rp_state* out);
+void rp_world_set_state(rp_world* w, const rp_state* in);
+uint32_t rp_world_error(rp_world* w, uint32_t scene);
+/* EPA runs that did not converge since creation (epa.cpp:233-236). */
+uint64_t rp_world_epa_failures(rp_world* w);
+/* The HIP stream the world's kernels run on (a hipStream_t). */
+void* rp_world_stream(rp_world* w);
+
+/* Inspection (used by parity tests): derived body data, broad-phase pairs
+ * and contacts at the current poses, hull topology of a body. Same
+ * semantics as the reference functions cited in the oracle's header. */
+void rp_world_body_info(rp_world* w, double* inverse_mass, double* inertia, double* inverse_inertia,
+                        double* bounding_radius);
+uint32_t rp_world_broad_pairs(rp_world* w, uint32_t scene, uint32_t* pairs, uint32_t cap);
+uint32_t rp_world_contacts(rp_world* w, uint32_t scene, uint32_t i, uint32_t j, double* out, uint32_t cap);
+void rp_world_hull(rp_world* w, uint32_t body, uint32_t counts[6], double* vertices, double* face_normals,
+                   uint32_t* face_off, uint32_t* face_el, uint32_t* vf_off, uint32_t* vf, uint32_t* vn_off,
+                   uint32_t* vn, uint32_t* ff_off, uint32_t* ff);
+
+/* Device FP64 primitives on host arrays of n inputs: out[5i..5i+4] =
+ * sqrt(a), a/b, asin(b/(|b|+1)), sin(b), cos(b). The step is bit-exact with
+ * the reference only if the device sqrt and division are correctly rounded;
+ * this entry point lets a test check that on the installed device. */
+uint32_t rp_fp64_selftest(const double* a, const double* b, double* out, uint64_t n);
+
+/* Last error of the calling thread: the error word of the last drop-in call
+ * and a message for runtime failures. */
+uint32_t rp_last_error(void);
+const char* rp_last_error_string(void);
+
+#ifdef __cplusplus
+}  /* extern "C" */
+
+#ifdef RP_DECLARE_REFERENCE_SYMBOLS
+/* The reference's own C++ signatures (src/physics/pbd.h:93-94), exported by
+ * librawphys.so. Declared only on request: a program that includes the
+ * reference headers already declares them. */
+struct Entity;
+struct Constraint;
+void pbd_simulate(double dt, Entity** entities, unsigned int num_substeps, unsigned int num_pos_iters,
+                  int enable_collisions);
+void pbd_simulate_with_constraints(double dt, Entity** entities, Constraint* external_constraints,
+                                   unsigned int num_substeps, unsigned int num_pos_iters, int enable_collisions);
+#endif
+#endif
+#endif

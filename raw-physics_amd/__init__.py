@@ -1,1 +1,120 @@
-"""raw-physics on MI355X: package placeholder (filled in by the product build)."""
+"""raw-physics on MI355X: the rigid-body XPBD step of felipeek/raw-physics as
+hand-written HIP kernels for gfx950, behind the C ABI in include/rawphys.h.
+
+Python is a thin ctypes layer over librawphys.so (built in-tree by build.py).
+There is no CPU fallback: creating a World without the built library, or
+without a HIP device, raises.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+from . import scenes
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE / "librawphys.so"
+_P = ctypes.c_void_p
+_lib = None
+
+
+def library():
+    """Loads librawphys.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python raw-physics_amd/build.py`")
+    lib = ctypes.CDLL(str(LIB_PATH))
+    u32, u64, f64, i32 = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_double, ctypes.c_int32
+    sig = {
+        "rp_world_create": (_P, [ctypes.POINTER(scenes.WorldDescC)]),
+        "rp_world_destroy": (None, [_P]),
+        "rp_world_num_bodies": (u32, [_P]),
+        "rp_world_num_scenes": (u32, [_P]),
+        "rp_world_step": (u32, [_P, f64, u32, u32, u32, i32, f64]),
+        "rp_world_step_async": (u32, [_P, f64, u32, u32, u32, i32, f64]),
+        "rp_world_synchronize": (u32, [_P]),
+        "rp_world_get_state": (None, [_P, ctypes.POINTER(scenes.StateC)]),
+        "rp_world_set_state": (None, [_P, ctypes.POINTER(scenes.StateC)]),
+        "rp_world_error": (u32, [_P, u32]),
+        "rp_world_epa_failures": (u64, [_P]),
+        "rp_world_stream": (_P, [_P]),
+        "rp_world_body_info": (None, [_P, _P, _P, _P, _P]),
+        "rp_world_broad_pairs": (u32, [_P, u32, _P, u32]),
+        "rp_world_contacts": (u32, [_P, u32, u32, u32, _P, u32]),
+        "rp_world_hull": (None, [_P, u32, _P] + [_P] * 10),
+        "rp_last_error": (u32, []),
+        "rp_last_error_string": (ctypes.c_char_p, []),
+        "rp_pbd_simulate": (u32, [f64, _P, _P, u32, u32, i32]),
+        "physics_simulate": (None, [_P, f64]),
+        "rp_fp64_selftest": (u32, [_P, _P, _P, u64]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def last_error_string() -> str:
+    return (library().rp_last_error_string() or b"").decode()
+
+
+class World:
+    """A batch of scenes resident on the GPU (rp_world_*)."""
+
+    def __init__(self, world: "scenes.World"):
+        lib = library()
+        self.lib = lib
+        self.world = world
+        self.desc, self._keep = world.desc()
+        self.h = lib.rp_world_create(ctypes.byref(self.desc))
+        if not self.h:
+            raise RuntimeError("rp_world_create failed: " + last_error_string())
+        self.nb = world.num_bodies
+        self.ns = world.num_scenes
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.rp_world_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, frames=1, dt=1.0 / 60.0, substeps=20, iters=1, collisions=True, gravity=10.0, sync=True):
+        f = self.lib.rp_world_step if sync else self.lib.rp_world_step_async
+        rc = f(self.h, dt, frames, substeps, iters, 1 if collisions else 0, gravity)
+        if rc:
+            raise RuntimeError("rp_world_step failed: " + last_error_string())
+
+    def synchronize(self):
+        if self.lib.rp_world_synchronize(self.h):
+            raise RuntimeError("rp_world_synchronize failed: " + last_error_string())
+
+    def state(self) -> dict:
+        st = scenes.alloc_state(self.nb)
+        s = scenes.state_struct(st)
+        self.lib.rp_world_get_state(self.h, ctypes.byref(s))
+        return st
+
+    def set_state(self, st: dict):
+        s = scenes.state_struct(st)
+        self.lib.rp_world_set_state(self.h, ctypes.byref(s))
+
+    def errors(self) -> np.ndarray:
+        return np.array([self.lib.rp_world_error(self.h, s) for s in range(self.ns)], dtype=np.uint32)
+
+    def epa_failures(self) -> int:
+        return int(self.lib.rp_world_epa_failures(self.h))
+
+    @property
+    def stream(self) -> int:
+        return int(self.lib.rp_world_stream(self.h) or 0)

@@ -1,0 +1,68 @@
+// rp_kernels.h — device-resident world layout and kernel launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "rp_step.h"
+
+namespace rp {
+
+enum { STAT_EPA_FAIL = 0, STAT_COUNT = 4 };
+
+// All device pointers of one world. Passed by value to every kernel.
+struct DevWorld {
+    int ns, nb, n_xf, narrow_blocks;
+    // scenes
+    const int32_t* scene_body_off;  // [ns+1]
+    const int32_t* scene_pair_base; // [ns+1]  candidate pair capacity prefix: n(n-1)/2 per scene
+    const int32_t* scene_ext_off;   // [ns+1]
+    const int32_t* scene_con_base;  // [ns+1]  constraint capacity prefix
+    // bodies
+    Body* body;                     // [nb]
+    M4* model;                      // [nb]  model matrix of the current substep
+    const int32_t* body_hull;       // [nb]
+    const int32_t* body_tv_off;     // [nb]
+    const int32_t* body_tn_off;     // [nb]
+    // hull instances (vertices scaled, local frame)
+    const int32_t *hull_nv, *hull_nf, *hull_v_off, *hull_fn_off, *hull_topo;
+    const V3* hull_v;
+    const V3* hull_fn;
+    const int32_t* topo_base;       // 8 offsets into topo_pool per topology
+    const int32_t* topo_pool;
+    // transformed geometry
+    V3* tv;
+    V3* tn;
+    const int2* xf;                 // [n_xf] (body, vertex) or (body, ~face)
+    // pairs
+    int2* pairs;                    // [pair capacity] scene-local (i, j)
+    int32_t* pair_count;            // [ns]
+    int32_t* live_local;            // [pair capacity] pair index within the scene
+    int32_t* live_count;            // [ns]
+    int32_t* live_prefix;           // [ns+1]
+    int2* live_global;              // [pair capacity] (scene, pair slot)
+    int32_t* ncontact;              // [pair capacity]
+    Contact* contacts;              // [pair capacity * MAXC]
+    // external constraints
+    const ExtC* ext;                // [n_ext]
+    double* ext_lambda;             // [3 n_ext]
+    // solver scratch
+    int32_t *con_ref, *con_level, *con_order, *con_lstart;
+    int32_t* body_last;             // [nb]
+    int32_t *uf_parent, *uf_flag;   // [nb]
+    // diagnostics
+    uint32_t* scene_err;            // [ns]
+    unsigned long long* stats;      // [STAT_COUNT]
+};
+
+struct FrameParams {
+    double dt = 0.0, h = 0.0, gravity = 0.0;
+    unsigned substeps = 0, iters = 0;
+    bool collisions = true;
+    bool forces_given = false;  // body F/T already hold the frame's force sums
+};
+
+hipError_t launch_frame(const DevWorld& w, const FrameParams& fp, hipStream_t st);
+hipError_t launch_pair_contacts(const DevWorld& w, int b1, int b2, double* out, int* count, uint32_t* err,
+                                hipStream_t st);
+hipError_t launch_broad(const DevWorld& w, hipStream_t st);
+hipError_t launch_fp64_selftest(const double* a, const double* b, double* out, int64_t n, hipStream_t st);
+
+}  // namespace rp

@@ -1,0 +1,804 @@
+// rp_step.h — per-pair and per-constraint device functions of the step.
+//
+// Each function restates one reference routine (file:line given) with the
+// same floating-point evaluation order, so that results are bit-identical
+// to the reference's CPU step. Memory is bounded: where the reference grows
+// heap arrays, these functions use fixed-capacity per-thread arrays and set
+// an error bit if a capacity would be exceeded.
+#pragma once
+#include <float.h>
+#include "rp_math.h"
+
+namespace rp {
+
+// ---------------------------------------------------------------------------
+// Device data layout
+// ---------------------------------------------------------------------------
+struct Body {            // one record per body, 464 bytes
+    V3 x; Q4 q; V3 v; V3 w;      // world_position, world_rotation, velocities
+    V3 px; Q4 pq; V3 pv; V3 pw;  // previous_* (entity.h:43-46)
+    double deact;                // deactivation_time
+    double inv_mass;
+    M3 I, Iinv;                  // local inertia tensor and its inverse
+    double mu_s, mu_d, e, radius;
+    V3 F, T;                     // external force / torque of the frame
+    int32_t fixed, active;
+};
+
+// A collision constraint as built by clipping_contact_to_collision_constraint
+// (pbd.cpp:408-424) plus its multipliers.
+struct Contact {
+    V3 r1, r2, n;
+    double lambda_n, lambda_t;
+};
+
+// External constraint (Constraint, pbd.h:22-91); b1/b2 are global body indices.
+struct ExtC {
+    int32_t type, b1, b2, limited;
+    int32_t axis[4];
+    V3 r1, r2, distance;
+    double compliance;
+    double limits[4];
+};
+
+// Hull topology and transformed geometry of one body.
+struct HullView {
+    const V3* tv;  // transformed vertices
+    const V3* tn;  // transformed face normals
+    int nv, nf;
+    const int32_t *fe_off, *fe, *vf_off, *vf, *vn_off, *vn, *ff_off, *ff;
+};
+
+enum : uint32_t {
+    ERR_GJK_SIMPLEX = 1u << 0,
+    ERR_EPA_DEGENERATE = 1u << 1,
+    ERR_SUPPORT_NAN = 1u << 2,
+    ERR_SKEW_LINES = 1u << 3,
+    ERR_ACTIVITY = 1u << 4,
+    ERR_HULL = 1u << 5,
+    ERR_ZERO_MASS = 1u << 6,
+    ERR_SELECT = 1u << 7,
+    ERR_EPA_CAPACITY = 1u << 8,
+    ERR_CLIP_CAPACITY = 1u << 9,
+    ERR_CONTACT_CAPACITY = 1u << 10,
+    ERR_SOLVER_CAPACITY = 1u << 11,
+};
+
+constexpr int EPA_MAXV = 104;  // 4 simplex points + at most 100 expansions (epa.cpp:149)
+constexpr int EPA_MAXF = 256;
+constexpr int EPA_MAXE = 128;
+constexpr int CLIP_MAX = 64;
+constexpr int MAXC = 16;       // contacts kept per body pair
+
+// ---------------------------------------------------------------------------
+// Support mapping (support.cpp:5-39): first index wins on ties
+// ---------------------------------------------------------------------------
+RP_HD int support_index(const HullView& h, V3 d, uint32_t& err) {
+    int sel = -1;
+    double max_dot = -DBL_MAX;
+    for (int i = 0; i < h.nv; ++i) {
+        double dd = dot(h.tv[i], d);
+        if (dd > max_dot) { sel = i; max_dot = dd; }
+    }
+    if (sel < 0) { err |= ERR_SUPPORT_NAN; sel = 0; }
+    return sel;
+}
+RP_HD V3 support_md(const HullView& a, const HullView& b, V3 d, uint32_t& err) {
+    V3 s1 = a.tv[support_index(a, d, err)];
+    V3 s2 = b.tv[support_index(b, scl(-1.0, d), err)];
+    return sub(s1, s2);
+}
+
+// ---------------------------------------------------------------------------
+// GJK (gjk.cpp:7-379)
+// ---------------------------------------------------------------------------
+struct Simplex { V3 a, b, c, d; int num; };
+
+RP_HD V3 triple_cross(V3 a, V3 b, V3 c) { return cross(cross(a, b), c); }
+
+// gjk.cpp:36-55
+RP_HD void do_simplex_2(Simplex& s, V3& dir) {
+    V3 a = s.a, b = s.b;
+    V3 ao = inv3(a), ab = sub(b, a);
+    if (dot(ab, ao) >= 0.0) { s.a = a; s.b = b; s.num = 2; dir = triple_cross(ab, ao, ab); }
+    else { s.a = a; s.num = 1; dir = ao; }
+}
+// gjk.cpp:57-120 ("A region" keeps num, gjk.cpp:81-85,95-98)
+RP_HD void do_simplex_3(Simplex& s, V3& dir) {
+    V3 a = s.a, b = s.b, c = s.c;
+    V3 ao = inv3(a), ab = sub(b, a), ac = sub(c, a), abc = cross(ab, ac);
+    if (dot(cross(abc, ac), ao) >= 0.0) {
+        if (dot(ac, ao) >= 0.0) { s.a = a; s.b = c; s.num = 2; dir = triple_cross(ac, ao, ac); }
+        else if (dot(ab, ao) >= 0.0) { s.a = a; s.b = b; s.num = 2; dir = triple_cross(ab, ao, ab); }
+        else { s.a = a; dir = ao; }
+    } else {
+        if (dot(cross(ab, abc), ao) >= 0.0) {
+            if (dot(ab, ao) >= 0.0) { s.a = a; s.b = b; s.num = 2; dir = triple_cross(ab, ao, ab); }
+            else { s.a = a; dir = ao; }
+        } else if (dot(abc, ao) >= 0.0) { s.a = a; s.b = b; s.c = c; s.num = 3; dir = abc; }
+        else { s.a = a; s.b = c; s.c = b; s.num = 3; dir = inv3(abc); }
+    }
+}
+// gjk.cpp:122-337 (the ACD/AC branch uses ab, gjk.cpp:213)
+RP_HD bool do_simplex_4(Simplex& s, V3& dir) {
+    V3 a = s.a, b = s.b, c = s.c, d = s.d;
+    V3 ao = inv3(a), ab = sub(b, a), ac = sub(c, a), ad = sub(d, a);
+    V3 abc = cross(ab, ac), acd = cross(ac, ad), adb = cross(ad, ab);
+    unsigned pi = 0;
+    if (dot(abc, ao) >= 0.0) pi |= 1;
+    if (dot(acd, ao) >= 0.0) pi |= 2;
+    if (dot(adb, ao) >= 0.0) pi |= 4;
+    switch (pi) {
+    case 0: return true;
+    case 1:
+        if (dot(cross(abc, ac), ao) >= 0.0) {
+            if (dot(ac, ao) >= 0.0) { s.a = a; s.b = c; s.num = 2; dir = triple_cross(ac, ao, ac); }
+            else if (dot(ab, ao) >= 0.0) { s.a = a; s.b = b; s.num = 2; dir = triple_cross(ab, ao, ab); }
+            else { s.a = a; dir = ao; }
+        } else {
+            if (dot(cross(ab, abc), ao) >= 0.0) {
+                if (dot(ab, ao) >= 0.0) { s.a = a; s.b = b; s.num = 2; dir = triple_cross(ab, ao, ab); }
+                else { s.a = a; dir = ao; }
+            } else { s.a = a; s.b = b; s.c = c; s.num = 3; dir = abc; }
+        }
+        break;
+    case 2:
+        if (dot(cross(acd, ad), ao) >= 0.0) {
+            if (dot(ad, ao) >= 0.0) { s.a = a; s.b = d; s.num = 2; dir = triple_cross(ad, ao, ad); }
+            else if (dot(ac, ao) >= 0.0) { s.a = a; s.b = c; s.num = 2; dir = triple_cross(ab, ao, ab); }
+            else { s.a = a; dir = ao; }
+        } else {
+            if (dot(cross(ac, acd), ao) >= 0.0) {
+                if (dot(ac, ao) >= 0.0) { s.a = a; s.b = c; s.num = 2; dir = triple_cross(ac, ao, ac); }
+                else { s.a = a; dir = ao; }
+            } else { s.a = a; s.b = c; s.c = d; s.num = 3; dir = acd; }
+        }
+        break;
+    case 3:
+        if (dot(ac, ao) >= 0.0) { s.a = a; s.b = c; s.num = 2; dir = triple_cross(ac, ao, ac); }
+        else { s.a = a; s.num = 1; dir = ao; }
+        break;
+    case 4:
+        if (dot(cross(adb, ab), ao) >= 0.0) {
+            if (dot(ab, ao) >= 0.0) { s.a = a; s.b = b; s.num = 2; dir = triple_cross(ab, ao, ab); }
+            else if (dot(ad, ao) >= 0.0) { s.a = a; s.b = d; s.num = 2; dir = triple_cross(ad, ao, ad); }
+            else { s.a = a; dir = ao; }
+        } else {
+            if (dot(cross(ad, adb), ao) >= 0.0) {
+                if (dot(ad, ao) >= 0.0) { s.a = a; s.b = d; s.num = 2; dir = triple_cross(ad, ao, ad); }
+                else { s.a = a; dir = ao; }
+            } else { s.a = a; s.b = d; s.c = b; s.num = 3; dir = adb; }
+        }
+        break;
+    case 5:
+        if (dot(ab, ao) >= 0.0) { s.a = a; s.b = b; s.num = 2; dir = triple_cross(ab, ao, ab); }
+        else { s.a = a; s.num = 1; dir = ao; }
+        break;
+    case 6:
+        if (dot(ad, ao) >= 0.0) { s.a = a; s.b = d; s.num = 2; dir = triple_cross(ad, ao, ad); }
+        else { s.a = a; s.num = 1; dir = ao; }
+        break;
+    case 7: s.a = a; s.num = 1; dir = ao; break;
+    }
+    return false;
+}
+
+// gjk_collides (gjk.cpp:350-379)
+RP_HD bool gjk(const HullView& h1, const HullView& h2, Simplex& s, uint32_t& err) {
+    s.a = support_md(h1, h2, v3(0.0, 0.0, 1.0), err);
+    s.b = s.c = s.d = v3(0.0, 0.0, 0.0);
+    s.num = 1;
+    V3 dir = scl(-1.0, s.a);
+    for (int i = 0; i < 100; ++i) {
+        V3 p = support_md(h1, h2, dir, err);
+        if (dot(p, dir) < 0.0) return false;
+        // add_to_simplex (gjk.cpp:7-30); num == 4 is an assert in the reference
+        switch (s.num) {
+        case 1: s.b = s.a; s.a = p; break;
+        case 2: s.c = s.b; s.b = s.a; s.a = p; break;
+        case 3: s.d = s.c; s.c = s.b; s.b = s.a; s.a = p; break;
+        default: err |= ERR_GJK_SIMPLEX; return false;
+        }
+        ++s.num;
+        bool hit = false;
+        if (s.num == 2) do_simplex_2(s, dir);
+        else if (s.num == 3) do_simplex_3(s, dir);
+        else hit = do_simplex_4(s, dir);
+        if (hit) return true;
+    }
+    return false;
+}
+
+// ---------------------------------------------------------------------------
+// EPA (epa.cpp:8-237)
+// ---------------------------------------------------------------------------
+struct I3 { int32_t x, y, z; };
+
+// get_face_normal_and_distance_to_origin (epa.cpp:32-77)
+RP_HD bool epa_face(I3 f, const V3* poly, int np, V3& n_out, double& d_out) {
+    V3 a = poly[f.x], b = poly[f.y], c = poly[f.z];
+    V3 n = nrm(cross(sub(b, a), sub(c, a)));
+    if (is_zero(n)) return false;
+    const double TOL = 0.0;
+    double d = dot(n, a);
+    if (d < -TOL) {
+        n = inv3(n);
+        d = -d;
+    } else if (d >= -TOL && d <= TOL) {
+        bool found = false;
+        for (int i = 0; i < np; ++i) {
+            double ad = dot(n, poly[i]);
+            if (ad < -TOL || ad > TOL) {
+                n = ad < -TOL ? n : inv3(n);
+                found = true;
+                break;
+            }
+        }
+        if (!found) return false;
+    }
+    n_out = n;
+    d_out = d;
+    return true;
+}
+
+// epa (epa.cpp:118-237): 0 = no convergence, 1 = converged, -1 = error
+RP_HD int epa(const HullView& h1, const HullView& h2, const Simplex& s, V3& n_out, double& pen_out, uint32_t& err) {
+    V3 poly[EPA_MAXV];
+    I3 faces[EPA_MAXF];
+    V3 normals[EPA_MAXF];
+    double dist[EPA_MAXF];
+    int2 edges[EPA_MAXE];
+    int np = 4, nf = 4, ne = 0;
+    poly[0] = s.a; poly[1] = s.b; poly[2] = s.c; poly[3] = s.d;
+    faces[0] = {0, 1, 2}; faces[1] = {0, 2, 3}; faces[2] = {0, 3, 1}; faces[3] = {1, 2, 3};
+    V3 min_n = v3(0.0, 0.0, 0.0);
+    double min_d = DBL_MAX;
+    for (int i = 0; i < nf; ++i) {
+        V3 n; double d;
+        if (!epa_face(faces[i], poly, np, n, d)) { err |= ERR_EPA_DEGENERATE; return -1; }
+        normals[i] = n;
+        dist[i] = d;
+        if (d < min_d) { min_d = d; min_n = n; }
+    }
+    for (int it = 0; it < 100; ++it) {
+        V3 sp = support_md(h1, h2, min_n, err);
+        double d = dot(min_n, sp);
+        if (fabs(d - min_d) < 0.0001) {
+            n_out = min_n;
+            pen_out = min_d;
+            return 1;
+        }
+        int new_idx = np;
+        poly[np++] = sp;
+        for (int i = 0; i < nf; ++i) {
+            V3 n = normals[i];
+            I3 f = faces[i];
+            // triangle_centroid (epa.cpp:112-116)
+            V3 c = scl(1.0 / 3.0, add(add(poly[f.y], poly[f.z]), poly[f.x]));
+            if (dot(n, sub(sp, c)) > 0.0) {
+                int2 es[3] = {make_int2(f.x, f.y), make_int2(f.y, f.z), make_int2(f.z, f.x)};
+                for (int k = 0; k < 3; ++k) {
+                    // add_edge (epa.cpp:79-110): cancel by index or by position, swap-remove
+                    int2 e = es[k];
+                    bool removed = false;
+                    for (int j = 0; j < ne; ++j) {
+                        int2 cur = edges[j];
+                        bool hit = (e.x == cur.x && e.y == cur.y) || (e.x == cur.y && e.y == cur.x);
+                        if (!hit) {
+                            V3 c1 = poly[cur.x], c2 = poly[cur.y], e1 = poly[e.x], e2 = poly[e.y];
+                            hit = (veq(c1, e1) && veq(c2, e2)) || (veq(c1, e2) && veq(c2, e1));
+                        }
+                        if (hit) {
+                            --ne;
+                            edges[j] = edges[ne];
+                            removed = true;
+                            break;
+                        }
+                    }
+                    if (!removed) {
+                        if (ne == EPA_MAXE) { err |= ERR_EPA_CAPACITY; return -1; }
+                        edges[ne++] = e;
+                    }
+                }
+                // swap-remove faces / distances / normals together (epa.cpp:191-195)
+                --nf;
+                faces[i] = faces[nf];
+                dist[i] = dist[nf];
+                normals[i] = normals[nf];
+                --i;
+            }
+        }
+        for (int i = 0; i < ne; ++i) {
+            if (nf == EPA_MAXF) { err |= ERR_EPA_CAPACITY; return -1; }
+            I3 f = {edges[i].x, edges[i].y, new_idx};
+            faces[nf] = f;
+            V3 n; double fd;
+            if (!epa_face(f, poly, np, n, fd)) { err |= ERR_EPA_DEGENERATE; return -1; }
+            normals[nf] = n;
+            dist[nf] = fd;
+            ++nf;
+        }
+        min_d = DBL_MAX;
+        for (int i = 0; i < nf; ++i)
+            if (dist[i] < min_d) { min_d = dist[i]; min_n = normals[i]; }
+        ne = 0;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Contact manifold (clipping.cpp:7-371)
+// ---------------------------------------------------------------------------
+struct Plane { V3 normal, point; };
+
+// is_point_in_plane (clipping.cpp:12-19): the plane offset is truncated to float
+RP_HD bool point_in_plane(const Plane& p, V3 pos) {
+    float distance = -dot(p.normal, p.point);
+    if (dot(pos, p.normal) + distance < 0.0) return false;
+    return true;
+}
+// plane_edge_intersection (clipping.cpp:21-47), with its float truncations
+RP_HD bool plane_edge(const Plane& p, V3 start, V3 end, V3& out) {
+    const double EPSILON = 0.000001;
+    V3 ab = sub(end, start);
+    float ab_p = dot(p.normal, ab);
+    if (fabs((double)ab_p) > EPSILON) {
+        float distance = -dot(p.normal, p.point);
+        V3 p_co = scl(-distance, p.normal);
+        float fac = -dot(p.normal, sub(start, p_co)) / ab_p;
+        fac = RP_MIN(RP_MAX(fac, 0.0), 1.0);
+        out = add(start, scl(fac, ab));
+        return true;
+    }
+    return false;
+}
+
+// One Sutherland-Hodgman pass against one plane (clipping.cpp:62-108).
+RP_HD int clip_pass(const V3* in, int n, const Plane& p, V3* out, bool remove, uint32_t& err) {
+    int m = 0;
+    if (n == 0) return 0;
+    V3 tmp, start = in[n - 1];
+    for (int j = 0; j < n; ++j) {
+        V3 end = in[j];
+        bool s_in = point_in_plane(p, start);
+        bool e_in = point_in_plane(p, end);
+        if (remove) {
+            if (e_in) { if (m < CLIP_MAX) out[m++] = end; else err |= ERR_CLIP_CAPACITY; }
+        } else if (s_in && e_in) {
+            if (m < CLIP_MAX) out[m++] = end; else err |= ERR_CLIP_CAPACITY;
+        } else if (s_in && !e_in) {
+            if (plane_edge(p, start, end, tmp)) { if (m < CLIP_MAX) out[m++] = tmp; else err |= ERR_CLIP_CAPACITY; }
+        } else if (!s_in && e_in) {
+            if (plane_edge(p, start, end, tmp)) { if (m < CLIP_MAX) out[m++] = tmp; else err |= ERR_CLIP_CAPACITY; }
+            if (m < CLIP_MAX) out[m++] = end; else err |= ERR_CLIP_CAPACITY;
+        }
+        start = end;
+    }
+    return m;
+}
+
+// get_face_with_most_fitting_normal (clipping.cpp:136-152)
+RP_HD int best_face(int sidx, const HullView& h, V3 n, uint32_t& err) {
+    double max_proj = -DBL_MAX;
+    int sel = -1;
+    for (int i = h.vf_off[sidx]; i < h.vf_off[sidx + 1]; ++i) {
+        double proj = dot(h.tn[h.vf[i]], n);
+        if (proj > max_proj) { max_proj = proj; sel = h.vf[i]; }
+    }
+    if (sel < 0) { err |= ERR_SELECT; sel = 0; }
+    return sel;
+}
+
+// collision_distance_between_skew_lines (clipping.cpp:210-239)
+RP_HD bool skew_lines(V3 p1, V3 d1, V3 p2, V3 d2, V3& l1, V3& l2) {
+    double n1 = d1.x * d2.x + d1.y * d2.y + d1.z * d2.z;
+    double n2 = d2.x * d2.x + d2.y * d2.y + d2.z * d2.z;
+    double m1 = -d1.x * d1.x - d1.y * d1.y - d1.z * d1.z;
+    double m2 = -d2.x * d1.x - d2.y * d1.y - d2.z * d1.z;
+    double r1 = -d1.x * p2.x + d1.x * p1.x - d1.y * p2.y + d1.y * p1.y - d1.z * p2.z + d1.z * p1.z;
+    double r2 = -d2.x * p2.x + d2.x * p1.x - d2.y * p2.y + d2.y * p1.y - d2.z * p2.z + d2.z * p1.z;
+    if ((n1 * m2) - (n2 * m1) == 0) return false;
+    double n = ((r1 * m2) - (r2 * m1)) / ((n1 * m2) - (n2 * m1));
+    double m = ((n1 * r2) - (n2 * r1)) / ((n1 * m2) - (n2 * m1));
+    l1 = add(p1, scl(m, d1));
+    l2 = add(p2, scl(n, d2));
+    return true;
+}
+
+struct ContactPt { V3 p1, p2, n; };
+
+// convex_convex_contact_manifold (clipping.cpp:249-341). Returns the number
+// of contacts written to `out` (at most MAXC).
+RP_HD int manifold(const HullView& h1, const HullView& h2, V3 normal, ContactPt* out, uint32_t& err) {
+    const double EPSILON = 0.0001;
+    V3 inormal = inv3(normal);
+    int s1 = support_index(h1, normal, err);
+    int s2 = support_index(h2, inormal, err);
+    int f1 = best_face(s1, h1, normal, err);
+    int f2 = best_face(s2, h2, inormal, err);
+    // get_edge_with_most_fitting_normal (clipping.cpp:154-200)
+    V3 edge_n = v3(0.0, 0.0, 0.0);
+    int e1a = 0, e1b = 0, e2a = 0, e2b = 0;
+    {
+        V3 sp1 = h1.tv[s1], sp2 = h2.tv[s2];
+        double max_dot = -DBL_MAX;
+        bool found = false;
+        for (int i = h1.vn_off[s1]; i < h1.vn_off[s1 + 1]; ++i) {
+            V3 ed1 = sub(sp1, h1.tv[h1.vn[i]]);
+            for (int j = h2.vn_off[s2]; j < h2.vn_off[s2 + 1]; ++j) {
+                V3 ed2 = sub(sp2, h2.tv[h2.vn[j]]);
+                V3 cn = nrm(cross(ed1, ed2));
+                V3 cni = inv3(cn);
+                double dd = dot(cn, normal);
+                if (dd > max_dot) { max_dot = dd; e1a = s1; e1b = h1.vn[i]; e2a = s2; e2b = h2.vn[j]; edge_n = cn; found = true; }
+                dd = dot(cni, normal);
+                if (dd > max_dot) { max_dot = dd; e1a = s1; e1b = h1.vn[i]; e2a = s2; e2b = h2.vn[j]; edge_n = cni; found = true; }
+            }
+        }
+        if (!found) err |= ERR_SELECT;
+    }
+    double d1 = dot(h1.tn[f1], normal);
+    double d2 = dot(h2.tn[f2], inormal);
+    double de = dot(edge_n, normal);
+    int nc = 0;
+    if (de > d1 + EPSILON && de > d2 + EPSILON) {
+        V3 l1, l2;
+        V3 p1 = h1.tv[e1a];
+        V3 dd1 = sub(h1.tv[e1b], p1);
+        V3 p2 = h2.tv[e2a];
+        V3 dd2 = sub(h2.tv[e2b], p2);
+        if (!skew_lines(p1, dd1, p2, dd2, l1, l2)) { err |= ERR_SKEW_LINES; return 0; }
+        out[0].p1 = l1; out[0].p2 = l2; out[0].n = normal;
+        return 1;
+    }
+    bool ref1 = d1 > d2;
+    const HullView& rh = ref1 ? h1 : h2;
+    const HullView& ih = ref1 ? h2 : h1;
+    int rf = ref1 ? f1 : f2, inf = ref1 ? f2 : f1;
+    V3 bufA[CLIP_MAX], bufB[CLIP_MAX];
+    int n = 0;
+    for (int i = ih.fe_off[inf]; i < ih.fe_off[inf + 1]; ++i) {
+        if (n < CLIP_MAX) bufA[n++] = ih.tv[ih.fe[i]]; else err |= ERR_CLIP_CAPACITY;
+    }
+    // clip against the reference face's boundary planes (clipping.cpp:121-134, 294)
+    V3* in = bufA;
+    V3* outp = bufB;
+    for (int k = rh.ff_off[rf]; k < rh.ff_off[rf + 1]; ++k) {
+        if (n == 0) break;
+        int nf = rh.ff[k];
+        Plane p;
+        p.point = rh.tv[rh.fe[rh.fe_off[nf]]];
+        p.normal = inv3(rh.tn[nf]);
+        n = clip_pass(in, n, p, outp, false, err);
+        V3* t = in; in = outp; outp = t;
+    }
+    // remove the points above the reference plane (clipping.cpp:296-302)
+    Plane rp;
+    rp.normal = inv3(rh.tn[rf]);
+    rp.point = rh.tv[rh.fe[rh.fe_off[rf]]];
+    n = clip_pass(in, n, rp, outp, true, err);
+    // get_closest_point_polygon (clipping.cpp:115-119) and contact points (clipping.cpp:304-329)
+    double pd = dot(scl(-1.0, rp.normal), rp.point);
+    for (int i = 0; i < n; ++i) {
+        V3 pt = outp[i];
+        V3 cp = sub(pt, scl(dot(rp.normal, pt) + pd, rp.normal));
+        V3 diff = sub(pt, cp);
+        double pen;
+        ContactPt c;
+        if (ref1) {
+            pen = dot(diff, normal);
+            c.p1 = sub(pt, scl(pen, normal));
+            c.p2 = pt;
+        } else {
+            pen = -dot(diff, normal);
+            c.p1 = pt;
+            c.p2 = add(pt, scl(pen, normal));
+        }
+        c.n = normal;
+        if (pen < 0.0) {
+            if (nc < MAXC) out[nc++] = c; else err |= ERR_CONTACT_CAPACITY;
+        }
+    }
+    return nc;
+}
+
+// collider_get_contacts for two hulls (collider.cpp:523-558): GJK, EPA, manifold.
+// `epa_fail` counts EPA runs that did not converge (the reference prints and
+// drops the pair's contacts, epa.cpp:233-236, collider.cpp:549-551).
+RP_HD int pair_contacts(const HullView& h1, const HullView& h2, ContactPt* out, uint32_t& err, int& epa_fail) {
+    Simplex s;
+    if (!gjk(h1, h2, s, err)) return 0;
+    V3 n; double pen;
+    int r = epa(h1, h2, s, n, pen, err);
+    if (r != 1) { epa_fail = 1; return 0; }
+    return manifold(h1, h2, n, out, err);
+}
+
+// ---------------------------------------------------------------------------
+// XPBD base constraints (pbd_base_constraints.cpp:6-227)
+// ---------------------------------------------------------------------------
+struct PosPrep { V3 r1, r2; M3 I1, I2; };
+
+RP_HD void pos_prep(const Body& e1, const Body& e2, V3 r1_lc, V3 r2_lc, PosPrep& p) {
+    p.r1 = q_rot(e1.q, r1_lc);
+    p.r2 = q_rot(e2.q, r2_lc);
+    p.I1 = world_tensor(e1.q, e1.Iinv);
+    p.I2 = world_tensor(e2.q, e2.Iinv);
+}
+RP_HD double pos_dlambda(const Body& e1, const Body& e2, const PosPrep& p, double h, double compliance, double lambda,
+                         V3 dx, uint32_t& err) {
+    double c = len(dx);
+    if (c <= 1e-50) return 0.0;
+    V3 n = v3(dx.x / c, dx.y / c, dx.z / c);
+    double w1 = e1.inv_mass + dot(cross(p.r1, n), m3_mulv(p.I1, cross(p.r1, n)));
+    double w2 = e2.inv_mass + dot(cross(p.r2, n), m3_mulv(p.I2, cross(p.r2, n)));
+    if (w1 + w2 == 0.0) err |= ERR_ZERO_MASS;
+    double tc = compliance / (h * h);
+    return (-c - tc * lambda) / (w1 + w2 + tc);
+}
+RP_HD void rot_update(Body& e, V3 a, bool plus) {
+    Q4 aq = {a.x, a.y, a.z, 0.0};
+    Q4 qq = q_mul(aq, e.q);
+    if (plus) {
+        e.q.x = e.q.x + 0.5 * qq.x; e.q.y = e.q.y + 0.5 * qq.y; e.q.z = e.q.z + 0.5 * qq.z; e.q.w = e.q.w + 0.5 * qq.w;
+    } else {
+        e.q.x = e.q.x - 0.5 * qq.x; e.q.y = e.q.y - 0.5 * qq.y; e.q.z = e.q.z - 0.5 * qq.z; e.q.w = e.q.w - 0.5 * qq.w;
+    }
+    e.q = q_nrm(e.q);
+}
+RP_HD void pos_apply(Body& e1, Body& e2, const PosPrep& p, double dl, V3 dx) {
+    double c = len(dx);
+    if (c <= 1e-50) return;
+    V3 n = v3(dx.x / c, dx.y / c, dx.z / c);
+    V3 imp = scl(dl, n);
+    if (!e1.fixed) e1.x = add(e1.x, scl(e1.inv_mass, imp));
+    if (!e2.fixed) e2.x = add(e2.x, scl(-e2.inv_mass, imp));
+    V3 a1 = m3_mulv(p.I1, cross(p.r1, imp));
+    V3 a2 = m3_mulv(p.I2, cross(p.r2, imp));
+    // quaternion products use the rotation before either body is updated
+    // (pbd_base_constraints.cpp:114-117); e1 and e2 are distinct bodies.
+    if (!e1.fixed) rot_update(e1, a1, true);
+    if (!e2.fixed) rot_update(e2, a2, false);
+}
+RP_HD double ang_dlambda(const M3& I1, const M3& I2, double h, double compliance, double lambda, V3 dq, uint32_t& err) {
+    double theta = len(dq);
+    if (theta <= 1e-50) return 0.0;
+    V3 n = v3(dq.x / theta, dq.y / theta, dq.z / theta);
+    double w1 = dot(n, m3_mulv(I1, n));
+    double w2 = dot(n, m3_mulv(I2, n));
+    if (w1 + w2 == 0.0) err |= ERR_ZERO_MASS;
+    double tc = compliance / (h * h);
+    return (-theta - tc * lambda) / (w1 + w2 + tc);
+}
+RP_HD void ang_apply(Body& e1, Body& e2, const M3& I1, const M3& I2, double dl, V3 dq) {
+    double theta = len(dq);
+    if (theta <= 1e-50) return;
+    V3 n = v3(dq.x / theta, dq.y / theta, dq.z / theta);
+    V3 imp = scl(-dl, n);
+    V3 a1 = m3_mulv(I1, imp);
+    V3 a2 = m3_mulv(I2, imp);
+    if (!e1.fixed) rot_update(e1, a1, true);
+    if (!e2.fixed) rot_update(e2, a2, false);
+}
+
+// collision_constraint_solve (pbd.cpp:107-154)
+RP_HD void solve_collision(Contact& c, Body& e1, Body& e2, double h, uint32_t& err) {
+    PosPrep p;
+    pos_prep(e1, e2, c.r1, c.r2, p);
+    V3 p1 = add(e1.x, p.r1);
+    V3 p2 = add(e2.x, p.r2);
+    double d = dot(sub(p1, p2), c.n);
+    if (d > 0.0) {
+        V3 dx = scl(d, c.n);
+        double dl = pos_dlambda(e1, e2, p, h, 0.0, c.lambda_n, dx, err);
+        pos_apply(e1, e2, p, dl, dx);
+        c.lambda_n += dl;
+        pos_prep(e1, e2, c.r1, c.r2, p);
+        p1 = add(e1.x, p.r1);
+        p2 = add(e2.x, p.r2);
+        dl = pos_dlambda(e1, e2, p, h, 0.0, c.lambda_t, dx, err);
+        const double mu_s = (e1.mu_s + e2.mu_s) / 2.0f;
+        double lambda_n = c.lambda_n;
+        double lambda_t = c.lambda_t + dl;
+        if (lambda_t > mu_s * lambda_n) {
+            V3 p1_til = add(e1.px, q_rot(e1.pq, c.r1));
+            V3 p2_til = add(e2.px, q_rot(e2.pq, c.r2));
+            V3 dp = sub(sub(p1, p1_til), sub(p2, p2_til));
+            V3 dp_t = sub(dp, scl(dot(dp, c.n), c.n));
+            pos_apply(e1, e2, p, dl, dp_t);
+            c.lambda_t += dl;
+        }
+    }
+}
+
+// limit_angle (pbd.cpp:175-217)
+RP_HD bool limit_angle(V3 n, V3 n1, V3 n2, double alpha, double beta, V3& dq) {
+    double phi = asin(dot(cross(n1, n2), n));
+    if (dot(n1, n2) < 0.0) phi = RP_PI_F - phi;
+    if (phi > RP_PI_F) phi = phi - 2.0 * RP_PI_F;
+    if (phi < -RP_PI_F) phi = phi + 2.0 * RP_PI_F;
+    if (phi < alpha || phi > beta) {
+        phi = RP_CLAMP(phi, alpha, beta);
+        Q4 rot = q_from_axis_radians(n, phi);
+        n1 = q_rot(rot, n1);
+        dq = cross(n1, n2);
+        return true;
+    }
+    return false;
+}
+
+// Joint and positional constraints (pbd.cpp:81-97, 156-173, 245-379).
+// lam[0..2] are the constraint's multipliers for this substep:
+//   positional: lambda; mutual orientation: lambda;
+//   hinge: aligned axes, position, limit; spherical: position, swing, twist.
+RP_HD void solve_external(const ExtC& c, double* lam, Body& e1, Body& e2, double h, uint32_t& err) {
+    bool bad = false;
+    switch (c.type) {
+    case 0: {  // POSITIONAL_CONSTRAINT
+        V3 dx = sub(sub(e1.x, e2.x), c.distance);
+        PosPrep p;
+        pos_prep(e1, e2, c.r1, c.r2, p);
+        double dl = pos_dlambda(e1, e2, p, h, c.compliance, lam[0], dx, err);
+        pos_apply(e1, e2, p, dl, dx);
+        lam[0] += dl;
+    } break;
+    case 2: {  // MUTUAL_ORIENTATION_CONSTRAINT
+        M3 I1 = world_tensor(e1.q, e1.Iinv), I2 = world_tensor(e2.q, e2.Iinv);
+        Q4 aux = q_mul(e1.q, q_inv(e2.q));
+        V3 dq = v3(2.0 * aux.x, 2.0 * aux.y, 2.0 * aux.z);
+        double dl = ang_dlambda(I1, I2, h, c.compliance, lam[0], dq, err);
+        ang_apply(e1, e2, I1, I2, dl, dq);
+        lam[0] += dl;
+    } break;
+    case 3: {  // HINGE_JOINT_CONSTRAINT
+        M3 I1 = world_tensor(e1.q, e1.Iinv), I2 = world_tensor(e2.q, e2.Iinv);
+        V3 a1 = q_axis(e1.q, c.axis[0], bad);
+        V3 a2 = q_axis(e2.q, c.axis[1], bad);
+        V3 dq = cross(a1, a2);
+        double dl = ang_dlambda(I1, I2, h, c.compliance, lam[0], dq, err);
+        ang_apply(e1, e2, I1, I2, dl, dq);
+        lam[0] += dl;
+        PosPrep p;
+        pos_prep(e1, e2, c.r1, c.r2, p);
+        V3 dx = sub(add(e1.x, p.r1), add(e2.x, p.r2));
+        dl = pos_dlambda(e1, e2, p, h, 0.0, lam[1], dx, err);
+        pos_apply(e1, e2, p, dl, dx);
+        lam[1] += dl;
+        if (c.limited) {
+            V3 n1 = q_axis(e1.q, c.axis[2], bad);
+            V3 n2 = q_axis(e2.q, c.axis[3], bad);
+            V3 n = q_axis(e1.q, c.axis[0], bad);
+            if (limit_angle(n, n1, n2, c.limits[0], c.limits[1], dq)) {
+                M3 J1 = world_tensor(e1.q, e1.Iinv), J2 = world_tensor(e2.q, e2.Iinv);
+                double d2 = ang_dlambda(J1, J2, h, 0.0, lam[2], dq, err);
+                ang_apply(e1, e2, J1, J2, d2, dq);
+                lam[2] += d2;
+            }
+        }
+    } break;
+    case 4: {  // SPHERICAL_JOINT_CONSTRAINT
+        const double EPSILON = 1e-50;
+        PosPrep p;
+        pos_prep(e1, e2, c.r1, c.r2, p);
+        V3 dx = sub(add(e1.x, p.r1), add(e2.x, p.r2));
+        double dl = pos_dlambda(e1, e2, p, h, 0.0, lam[0], dx, err);
+        pos_apply(e1, e2, p, dl, dx);
+        lam[0] += dl;
+        V3 n1 = q_axis(e1.q, c.axis[0], bad);
+        V3 n2 = q_axis(e2.q, c.axis[1], bad);
+        V3 n = cross(n1, n2);
+        double n_len = len(n);
+        if (n_len > EPSILON) {
+            n = v3(n.x / n_len, n.y / n_len, n.z / n_len);
+            V3 dq;
+            if (limit_angle(n, n1, n2, c.limits[0], c.limits[1], dq)) {
+                M3 I1 = world_tensor(e1.q, e1.Iinv), I2 = world_tensor(e2.q, e2.Iinv);
+                double d2 = ang_dlambda(I1, I2, h, 0.0, lam[1], dq, err);
+                ang_apply(e1, e2, I1, I2, d2, dq);
+                lam[1] += d2;
+            }
+        }
+        V3 a1 = q_axis(e1.q, c.axis[0], bad);
+        V3 b1 = q_axis(e1.q, c.axis[2], bad);
+        V3 a2 = q_axis(e2.q, c.axis[1], bad);
+        V3 b2 = q_axis(e2.q, c.axis[3], bad);
+        n = add(a1, a2);
+        n_len = len(n);
+        if (n_len > EPSILON) {
+            n = v3(n.x / n_len, n.y / n_len, n.z / n_len);
+            n1 = sub(b1, scl(dot(n, b1), n));
+            n2 = sub(b2, scl(dot(n, b2), n));
+            double n1_len = len(n1);
+            double n2_len = len(n2);
+            if (n1_len > EPSILON && n2_len > EPSILON) {
+                n1 = v3(n1.x / n1_len, n1.y / n1_len, n1.z / n1_len);
+                n2 = v3(n2.x / n2_len, n2.y / n2_len, n2.z / n2_len);
+                V3 dq;
+                if (limit_angle(n, n1, n2, c.limits[2], c.limits[3], dq)) {
+                    M3 I1 = world_tensor(e1.q, e1.Iinv), I2 = world_tensor(e2.q, e2.Iinv);
+                    double d3 = ang_dlambda(I1, I2, h, 0.0, lam[2], dq, err);
+                    ang_apply(e1, e2, I1, I2, d3, dq);
+                    lam[2] += d3;
+                }
+            }
+        }
+    } break;
+    default: bad = true; break;
+    }
+    if (bad) err |= ERR_SELECT;
+}
+
+// Velocity solve of one collision constraint (pbd.cpp:646-711)
+RP_HD void velocity_solve(const Contact& c, Body& e1, Body& e2, double h) {
+    V3 n = c.n;
+    double lambda_n = c.lambda_n;
+    PosPrep p;
+    pos_prep(e1, e2, c.r1, c.r2, p);
+    V3 v = sub(add(e1.v, cross(e1.w, p.r1)), add(e2.v, cross(e2.w, p.r2)));
+    double vn = dot(n, v);
+    V3 vt = sub(v, scl(vn, n));
+    V3 dv = v3(0.0, 0.0, 0.0);
+    const double mu_d = (e1.mu_d + e2.mu_d) / 2.0f;
+    double fn = lambda_n / h;
+    double fact = RP_MIN(mu_d * fabs(fn), len(vt));
+    dv = add(dv, scl(-fact, nrm(vt)));
+    V3 vtil = sub(add(e1.pv, cross(e1.pw, p.r1)), add(e2.pv, cross(e2.pw, p.r2)));
+    double vn_til = dot(n, vtil);
+    double e = e1.e * e2.e;
+    fact = -vn + RP_MIN(-e * vn_til, 0.0);
+    dv = add(dv, scl(fact, n));
+    double w1 = e1.inv_mass + dot(cross(p.r1, n), m3_mulv(p.I1, cross(p.r1, n)));
+    double w2 = e2.inv_mass + dot(cross(p.r2, n), m3_mulv(p.I2, cross(p.r2, n)));
+    V3 pimp = scl(1.0 / (w1 + w2), dv);
+    if (!e1.fixed) {
+        e1.v = add(e1.v, scl(e1.inv_mass, pimp));
+        e1.w = add(e1.w, m3_mulv(p.I1, cross(p.r1, pimp)));
+    }
+    if (!e2.fixed) {
+        e2.v = add(e2.v, inv3(scl(e2.inv_mass, pimp)));
+        e2.w = add(e2.w, inv3(m3_mulv(p.I2, cross(p.r2, pimp))));
+    }
+}
+
+// Integration of one body (pbd.cpp:537-577)
+RP_HD void integrate(Body& e, double h) {
+    e.px = e.x;
+    e.pq = e.q;
+    if (e.fixed || !e.active) return;
+    e.v = add(e.v, scl(h * e.inv_mass, e.F));
+    e.x = add(e.x, scl(h, e.v));
+    M3 Iinv = world_tensor(e.q, e.Iinv);
+    M3 I = world_tensor(e.q, e.I);
+    e.w = add(e.w, scl(h, m3_mulv(Iinv, sub(e.T, cross(e.w, m3_mulv(I, e.w))))));
+    Q4 aux = {e.w.x, e.w.y, e.w.z, 0.0};
+    Q4 q = q_mul(aux, e.q);
+    e.q.x = e.q.x + h * 0.5 * q.x;
+    e.q.y = e.q.y + h * 0.5 * q.y;
+    e.q.z = e.q.z + h * 0.5 * q.z;
+    e.q.w = e.q.w + h * 0.5 * q.w;
+    e.q = q_nrm(e.q);
+}
+
+// Velocity update of one body (pbd.cpp:623-643)
+RP_HD void velocity_update(Body& e, double h) {
+    if (e.fixed || !e.active) return;
+    e.pv = e.v;
+    e.pw = e.w;
+    e.v = scl(1.0 / h, sub(e.x, e.px));
+    Q4 dq = q_mul(e.q, q_inv(e.pq));
+    if (dq.w >= 0.0) e.w = scl(2.0 / h, v3(dq.x, dq.y, dq.z));
+    else e.w = scl(-2.0 / h, v3(dq.x, dq.y, dq.z));
+}
+
+// clipping_contact_to_collision_constraint (pbd.cpp:408-424)
+RP_HD void contact_to_constraint(const Body& e1, const Body& e2, const ContactPt& ct, Contact& c) {
+    c.n = ct.n;
+    c.lambda_n = 0.0;
+    c.lambda_t = 0.0;
+    V3 r1w = sub(ct.p1, e1.x);
+    V3 r2w = sub(ct.p2, e2.x);
+    c.r1 = q_rot(q_inv(e1.q), r1w);
+    c.r2 = q_rot(q_inv(e2.q), r2w);
+}
+
+}  // namespace rp

@@ -1,0 +1,977 @@
+// rp_world.cpp — host runtime of librawphys.so: hull construction, world
+// upload to HBM, frame stepping, state I/O and the drop-in pbd_simulate.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/rawphys.h"
+#include "rp_kernels.h"
+
+namespace rp {
+
+static thread_local uint32_t g_last_err = 0;
+static thread_local std::string g_last_msg;
+
+static void set_msg(const std::string& m) { g_last_msg = m; }
+
+#define RP_CHECK(expr)                                                                    \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess) {                                                           \
+            set_msg(std::string(#expr) + ": " + hipGetErrorString(e_));                   \
+            return false;                                                                 \
+        }                                                                                 \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Hull construction (collider_convex_hull_create, collider.cpp:194-364)
+// ---------------------------------------------------------------------------
+struct HostHull {
+    std::vector<V3> v, fn;
+    std::vector<int32_t> fe_off, fe, vf_off, vf, vn_off, vn, ff_off, ff;
+    bool ok = true;
+};
+
+struct Tri { int32_t x, y, z; };
+struct Edge { int32_t x, y; };
+
+static void flatten(const std::vector<std::vector<int32_t>>& lists, std::vector<int32_t>& off, std::vector<int32_t>& d) {
+    off.assign(1, 0);
+    d.clear();
+    for (const auto& l : lists) {
+        d.insert(d.end(), l.begin(), l.end());
+        off.push_back((int32_t)d.size());
+    }
+}
+
+// collect_faces_planar_to (collider.cpp:47-77): depth-first over triangle
+// neighbours, gathering triangles whose normal matches `target`.
+static void planar_flood(const std::vector<V3>& hv, const std::vector<Tri>& tris,
+                         const std::vector<std::vector<int32_t>>& tri_nb, std::vector<char>& done, int32_t f, V3 target,
+                         std::vector<Tri>& out) {
+    const double EPSILON = 0.000001;
+    Tri t = tris[f];
+    V3 n = nrm(cross(sub(hv[t.y], hv[t.x]), sub(hv[t.z], hv[t.x])));
+    if (done[f]) return;
+    double projection = dot(n, target);
+    if ((projection - 1.0) > -EPSILON && (projection - 1.0) < EPSILON) {
+        out.push_back(t);
+        done[f] = 1;
+        for (int32_t nb : tri_nb[f]) planar_flood(hv, tris, tri_nb, done, nb, target, out);
+    }
+}
+
+// create_convex_hull_face (collider.cpp:93-166): the boundary loop of a set
+// of coplanar triangles, in the reference's edge order.
+static bool face_elements(const std::vector<Tri>& tris, std::vector<int32_t>& el) {
+    std::vector<Edge> edges;
+    auto find = [&](Edge q) -> int {
+        for (size_t i = 0; i < edges.size(); ++i)
+            if ((edges[i].x == q.x && edges[i].y == q.y) || (edges[i].x == q.y && edges[i].y == q.x)) return (int)i;
+        return -1;
+    };
+    for (const Tri& t : tris) {
+        Edge es[3] = {{t.x, t.y}, {t.y, t.z}, {t.z, t.x}};
+        for (Edge e : es) {
+            int i = find(e);
+            if (i >= 0) {  // array_remove: swap with last (light_array.h:146)
+                edges[i] = edges.back();
+                edges.pop_back();
+            } else {
+                edges.push_back(e);
+            }
+        }
+    }
+    for (size_t i = 0; i < edges.size(); ++i) {
+        Edge cur = edges[i];
+        for (size_t j = i + 1; j < edges.size(); ++j) {
+            Edge cand = edges[j];
+            if (cur.y != cand.x && cur.y != cand.y) continue;
+            if (cur.y == cand.y) std::swap(cand.x, cand.y);
+            Edge tmp = edges[i + 1];
+            edges[i + 1] = cand;
+            edges[j] = tmp;
+        }
+    }
+    el.clear();
+    for (const Edge& e : edges) el.push_back(e.x);
+    return !edges.empty() && edges.front().x == edges.back().y;
+}
+
+static HostHull build_hull(const std::vector<V3>& corners) {
+    HostHull H;
+    // unique vertices in first-appearance order (collider.cpp:200-209);
+    // keys compare with ==, so the two zeros are one key
+    struct Key {
+        uint64_t a, b, c;
+        bool operator==(const Key& o) const { return a == o.a && b == o.b && c == o.c; }
+    };
+    struct KeyHash {
+        size_t operator()(const Key& k) const { return (size_t)(k.a * 0x9E3779B97F4A7C15ull ^ (k.b + 0x632BE59BD9B4E019ull) * 31 ^ k.c); }
+    };
+    auto key = [](V3 p) {
+        double x = p.x + 0.0, y = p.y + 0.0, z = p.z + 0.0;
+        Key k;
+        memcpy(&k.a, &x, 8); memcpy(&k.b, &y, 8); memcpy(&k.c, &z, 8);
+        return k;
+    };
+    std::unordered_map<Key, int32_t, KeyHash> index;
+    std::vector<int32_t> cidx(corners.size());
+    for (size_t c = 0; c < corners.size(); ++c) {
+        Key k = key(corners[c]);
+        auto it = index.find(k);
+        if (it == index.end()) {
+            it = index.emplace(k, (int32_t)H.v.size()).first;
+            H.v.push_back(corners[c]);
+        }
+        cidx[c] = it->second;
+    }
+    size_t nt = corners.size() / 3, nv = H.v.size();
+    std::vector<Tri> tris(nt);
+    for (size_t i = 0; i < nt; ++i) tris[i] = {cidx[3 * i], cidx[3 * i + 1], cidx[3 * i + 2]};
+    // triangle neighbours and vertex neighbours (collider.cpp:248-282)
+    std::vector<std::vector<int32_t>> tri_nb(nt), vn(nv), vf(nv);
+    auto add_unique = [](std::vector<int32_t>& l, int32_t v) {
+        if (std::find(l.begin(), l.end(), v) == l.end()) l.push_back(v);
+    };
+    for (size_t i = 0; i < nt; ++i) {
+        Tri t = tris[i];
+        for (size_t j = 0; j < nt; ++j) {
+            if (i == j) continue;
+            Tri u = tris[j];
+            bool share = t.x == u.x || t.x == u.y || t.x == u.z || t.y == u.x || t.y == u.y || t.y == u.z ||
+                         t.z == u.x || t.z == u.y || t.z == u.z;
+            if (share) tri_nb[i].push_back((int32_t)j);
+        }
+        add_unique(vn[t.x], t.y); add_unique(vn[t.x], t.z);
+        add_unique(vn[t.y], t.x); add_unique(vn[t.y], t.z);
+        add_unique(vn[t.z], t.x); add_unique(vn[t.z], t.y);
+    }
+    // merge coplanar triangles into faces (collider.cpp:284-319)
+    std::vector<char> done(nt, 0);
+    std::vector<std::vector<int32_t>> felems;
+    for (size_t i = 0; i < nt; ++i) {
+        if (done[i]) continue;
+        Tri t = tris[i];
+        V3 n = nrm(cross(sub(H.v[t.y], H.v[t.x]), sub(H.v[t.z], H.v[t.x])));
+        std::vector<Tri> planar;
+        planar_flood(H.v, tris, tri_nb, done, (int32_t)i, n, planar);
+        std::vector<int32_t> el;
+        H.ok &= face_elements(planar, el);
+        int32_t fi = (int32_t)felems.size();
+        felems.push_back(el);
+        H.fn.push_back(n);
+        for (const Tri& p : planar) { vf[p.x].push_back(fi); vf[p.y].push_back(fi); vf[p.z].push_back(fi); }
+    }
+    // face neighbours: faces sharing any vertex (collider.cpp:321-341)
+    size_t nf = felems.size();
+    std::vector<std::vector<int32_t>> ff(nf);
+    for (size_t i = 0; i < nf; ++i)
+        for (size_t j = 0; j < nf; ++j) {
+            if (i == j) continue;
+            bool share = false;
+            for (int32_t a : felems[i]) {
+                for (int32_t b : felems[j])
+                    if (a == b) { share = true; break; }
+                if (share) break;
+            }
+            if (share) ff[i].push_back((int32_t)j);
+        }
+    flatten(felems, H.fe_off, H.fe);
+    flatten(vf, H.vf_off, H.vf);
+    flatten(vn, H.vn_off, H.vn);
+    flatten(ff, H.ff_off, H.ff);
+    H.ok &= nv > 0 && nf > 0;
+    return H;
+}
+
+// Point-mass inertia at the hull vertices (colliders_get_default_inertia_tensor,
+// collider.cpp:448-494) and its inverse (gm_mat3_inverse, gm.h:347-368).
+static M3 hull_inertia(const HostHull& h, double mass) {
+    double mpv = mass / (uint32_t)h.v.size();
+    M3 I;
+    memset(&I, 0, sizeof(I));
+    for (const V3& v : h.v) {
+        I.m[0][0] += mpv * (v.y * v.y + v.z * v.z);
+        I.m[0][1] += mpv * v.x * v.y;
+        I.m[0][2] += mpv * v.x * v.z;
+        I.m[1][0] += mpv * v.x * v.y;
+        I.m[1][1] += mpv * (v.x * v.x + v.z * v.z);
+        I.m[1][2] += mpv * v.y * v.z;
+        I.m[2][0] += mpv * v.x * v.z;
+        I.m[2][1] += mpv * v.y * v.z;
+        I.m[2][2] += mpv * (v.x * v.x + v.y * v.y);
+    }
+    return I;
+}
+static bool m3_inverse(const M3& M, M3& out) {
+    const double(*a)[3] = M.m;
+    double det = a[0][0] * (a[1][1] * a[2][2] - a[2][1] * a[1][2]) - a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
+                 a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+    if (det == 0.0) return false;
+    double invdet = 1 / det;
+    out.m[0][0] = (a[1][1] * a[2][2] - a[2][1] * a[1][2]) * invdet;
+    out.m[0][1] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) * invdet;
+    out.m[0][2] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) * invdet;
+    out.m[1][0] = (a[1][2] * a[2][0] - a[1][0] * a[2][2]) * invdet;
+    out.m[1][1] = (a[0][0] * a[2][2] - a[0][2] * a[2][0]) * invdet;
+    out.m[1][2] = (a[1][0] * a[0][2] - a[0][0] * a[1][2]) * invdet;
+    out.m[2][0] = (a[1][0] * a[2][1] - a[2][0] * a[1][1]) * invdet;
+    out.m[2][1] = (a[2][0] * a[0][1] - a[0][0] * a[2][1]) * invdet;
+    out.m[2][2] = (a[0][0] * a[1][1] - a[1][0] * a[0][1]) * invdet;
+    return true;
+}
+static double hull_radius(const HostHull& h) {  // collider.cpp:178-189, 510-521
+    double maxd = 0.0;
+    for (const V3& v : h.v) {
+        double l = len(v);
+        if (l > maxd) maxd = l;
+    }
+    return maxd;
+}
+
+// ---------------------------------------------------------------------------
+// Host-side world and its device image
+// ---------------------------------------------------------------------------
+struct HostWorld {
+    std::vector<int32_t> scene_body_off{0}, scene_ext_off{0};
+    std::vector<Body> body;
+    std::vector<int32_t> body_hull;
+    std::vector<HostHull> hulls;
+    std::vector<ExtC> ext;
+    std::vector<uint32_t> scene_err;  // errors found while building
+};
+
+struct DeviceImage {
+    DevWorld d;
+    std::vector<void*> allocs;
+    std::vector<int32_t> body_tv_off, body_tn_off;  // host copies for inspection
+    bool ok = false;
+
+    template <class T>
+    bool alloc(T*& p, size_t n) {
+        void* q = nullptr;
+        size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+        RP_CHECK(hipMalloc(&q, bytes));
+        RP_CHECK(hipMemset(q, 0, bytes));
+        allocs.push_back(q);
+        p = (T*)q;
+        return true;
+    }
+    template <class T>
+    bool upload(const T*& p, const std::vector<T>& h) {
+        T* q;
+        if (!alloc(q, h.size())) return false;
+        if (!h.empty()) RP_CHECK(hipMemcpy(q, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+        p = q;
+        return true;
+    }
+    template <class T>
+    bool upload_mut(T*& p, const std::vector<T>& h) {
+        if (!alloc(p, h.size())) return false;
+        if (!h.empty()) RP_CHECK(hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+        return true;
+    }
+    void release() {
+        for (void* p : allocs) (void)hipFree(p);
+        allocs.clear();
+    }
+};
+
+static bool build_image(const HostWorld& hw, DeviceImage& im) {
+    DevWorld& d = im.d;
+    memset(&d, 0, sizeof(d));
+    d.ns = (int)hw.scene_body_off.size() - 1;
+    d.nb = (int)hw.body.size();
+    // topology dedupe: bodies of the same shape share index lists
+    std::map<std::vector<int32_t>, int32_t> topo_ids;
+    std::vector<int32_t> topo_base, topo_pool;
+    std::vector<int32_t> hull_nv, hull_nf, hull_v_off, hull_fn_off, hull_topo;
+    std::vector<V3> hull_v, hull_fn;
+    for (const HostHull& h : hw.hulls) {
+        std::vector<int32_t> key;
+        const std::vector<int32_t>* parts[8] = {&h.fe_off, &h.fe, &h.vf_off, &h.vf, &h.vn_off, &h.vn, &h.ff_off, &h.ff};
+        for (auto* p : parts) {
+            key.push_back((int32_t)p->size());
+            key.insert(key.end(), p->begin(), p->end());
+        }
+        auto it = topo_ids.find(key);
+        if (it == topo_ids.end()) {
+            int32_t id = (int32_t)topo_ids.size();
+            for (auto* p : parts) {
+                topo_base.push_back((int32_t)topo_pool.size());
+                topo_pool.insert(topo_pool.end(), p->begin(), p->end());
+            }
+            it = topo_ids.emplace(key, id).first;
+        }
+        hull_topo.push_back(it->second);
+        hull_nv.push_back((int32_t)h.v.size());
+        hull_nf.push_back((int32_t)h.fn.size());
+        hull_v_off.push_back((int32_t)hull_v.size());
+        hull_fn_off.push_back((int32_t)hull_fn.size());
+        hull_v.insert(hull_v.end(), h.v.begin(), h.v.end());
+        hull_fn.insert(hull_fn.end(), h.fn.begin(), h.fn.end());
+    }
+    // per-body transformed geometry and the transform work list
+    std::vector<int2> xf;
+    int64_t ntv = 0, ntn = 0;
+    im.body_tv_off.resize(d.nb);
+    im.body_tn_off.resize(d.nb);
+    for (int b = 0; b < d.nb; ++b) {
+        const HostHull& h = hw.hulls[hw.body_hull[b]];
+        im.body_tv_off[b] = (int32_t)ntv;
+        im.body_tn_off[b] = (int32_t)ntn;
+        for (size_t i = 0; i < h.v.size(); ++i) xf.push_back(make_int2(b, (int)i));
+        for (size_t f = 0; f < h.fn.size(); ++f) xf.push_back(make_int2(b, ~(int)f));
+        ntv += (int64_t)h.v.size();
+        ntn += (int64_t)h.fn.size();
+    }
+    d.n_xf = (int)xf.size();
+    // pair and constraint capacities per scene
+    std::vector<int32_t> pair_base(d.ns + 1, 0), con_base(d.ns + 1, 0);
+    int64_t pcap = 0, ccap = 0;
+    for (int s = 0; s < d.ns; ++s) {
+        int64_t n = hw.scene_body_off[s + 1] - hw.scene_body_off[s];
+        int64_t next = hw.scene_ext_off[s + 1] - hw.scene_ext_off[s];
+        pair_base[s] = (int32_t)pcap;
+        con_base[s] = (int32_t)ccap;
+        pcap += n * (n - 1) / 2;
+        ccap += next + (n * (n - 1) / 2) * MAXC;
+        if (pcap > INT32_MAX / MAXC || ccap > INT32_MAX - d.ns - 1) {
+            set_msg("world too large for 32-bit pair/constraint indices");
+            return false;
+        }
+    }
+    pair_base[d.ns] = (int32_t)pcap;
+    con_base[d.ns] = (int32_t)ccap;
+    d.narrow_blocks = (int)std::max<int64_t>(1, std::min<int64_t>((pcap + 63) / 64, 8192));
+
+    bool ok = true;
+    ok &= im.upload(d.scene_body_off, hw.scene_body_off);
+    ok &= im.upload(d.scene_pair_base, pair_base);
+    ok &= im.upload(d.scene_ext_off, hw.scene_ext_off);
+    ok &= im.upload(d.scene_con_base, con_base);
+    ok &= im.upload_mut(d.body, hw.body);
+    ok &= im.alloc(d.model, d.nb);
+    ok &= im.upload(d.body_hull, hw.body_hull);
+    const int32_t* p;
+    ok &= im.upload(p, im.body_tv_off); d.body_tv_off = p;
+    ok &= im.upload(p, im.body_tn_off); d.body_tn_off = p;
+    ok &= im.upload(d.hull_nv, hull_nv);
+    ok &= im.upload(d.hull_nf, hull_nf);
+    ok &= im.upload(d.hull_v_off, hull_v_off);
+    ok &= im.upload(d.hull_fn_off, hull_fn_off);
+    ok &= im.upload(d.hull_topo, hull_topo);
+    ok &= im.upload(d.hull_v, hull_v);
+    ok &= im.upload(d.hull_fn, hull_fn);
+    ok &= im.upload(d.topo_base, topo_base);
+    ok &= im.upload(d.topo_pool, topo_pool);
+    ok &= im.alloc(d.tv, (size_t)ntv);
+    ok &= im.alloc(d.tn, (size_t)ntn);
+    ok &= im.upload(d.xf, xf);
+    ok &= im.alloc(d.pairs, (size_t)pcap);
+    ok &= im.alloc(d.pair_count, d.ns);
+    ok &= im.alloc(d.live_local, (size_t)pcap);
+    ok &= im.alloc(d.live_count, d.ns);
+    ok &= im.alloc(d.live_prefix, d.ns + 1);
+    ok &= im.alloc(d.live_global, (size_t)pcap);
+    ok &= im.alloc(d.ncontact, (size_t)pcap);
+    ok &= im.alloc(d.contacts, (size_t)pcap * MAXC);
+    ok &= im.upload(d.ext, hw.ext);
+    ok &= im.alloc(d.ext_lambda, 3 * hw.ext.size());
+    ok &= im.alloc(d.con_ref, (size_t)ccap);
+    ok &= im.alloc(d.con_level, (size_t)ccap);
+    ok &= im.alloc(d.con_order, (size_t)ccap);
+    ok &= im.alloc(d.con_lstart, (size_t)ccap + d.ns);
+    ok &= im.alloc(d.body_last, d.nb);
+    ok &= im.alloc(d.uf_parent, d.nb);
+    ok &= im.alloc(d.uf_flag, d.nb);
+    uint32_t* errp;
+    ok &= im.alloc(errp, d.ns);
+    d.scene_err = errp;
+    if (ok && !hw.scene_err.empty())
+        ok &= hipMemcpy(errp, hw.scene_err.data(), d.ns * sizeof(uint32_t), hipMemcpyHostToDevice) == hipSuccess;
+    ok &= im.alloc(d.stats, STAT_COUNT);
+    im.ok = ok;
+    return ok;
+}
+
+// Entity fields at creation (entity_create_ex, entity.cpp:24-65).
+static Body make_body(const HostHull& h, V3 x, Q4 q, V3 v, V3 w, double mass, bool fixed, const double mat[3],
+                      uint32_t& err) {
+    Body e;
+    memset(&e, 0, sizeof(e));
+    e.x = x; e.q = q; e.v = v; e.w = w;
+    e.radius = hull_radius(h);
+    e.fixed = fixed ? 1 : 0;
+    if (!fixed) {
+        e.inv_mass = 1.0 / mass;
+        e.I = hull_inertia(h, mass);
+        if (!m3_inverse(e.I, e.Iinv)) err |= ERR_HULL;
+    }
+    e.active = 1;
+    e.deact = 0.0;
+    e.mu_s = mat[0];
+    e.mu_d = mat[1];
+    e.e = mat[2];
+    return e;
+}
+
+static bool host_world_from_desc(const rp_world_desc* d, HostWorld& hw) {
+    uint32_t ns = d->num_scenes;
+    uint32_t nb = d->scene_body_offset[ns];
+    hw.scene_err.assign(ns, 0);
+    // hull instances: one per distinct (shape, scale)
+    std::map<std::tuple<uint32_t, uint64_t, uint64_t, uint64_t>, int32_t> inst;
+    for (uint32_t s = 0; s < ns; ++s) {
+        for (uint32_t g = d->scene_body_offset[s]; g < d->scene_body_offset[s + 1]; ++g) {
+            uint32_t shp = d->body_shape[g];
+            if (shp >= d->num_shapes) { set_msg("body_shape out of range"); return false; }
+            const double* sc = d->body_scale + 3 * g;
+            uint64_t k0, k1, k2;
+            memcpy(&k0, sc, 8); memcpy(&k1, sc + 1, 8); memcpy(&k2, sc + 2, 8);
+            auto key = std::make_tuple(shp, k0, k1, k2);
+            auto it = inst.find(key);
+            if (it == inst.end()) {
+                std::vector<V3> corners;
+                for (uint32_t c = d->shape_corner_offset[shp]; c < d->shape_corner_offset[shp + 1]; ++c) {
+                    // examples_util.cpp:7-16: float widened to double, then scaled
+                    V3 p = v3((double)d->shape_corners[3 * c], (double)d->shape_corners[3 * c + 1],
+                              (double)d->shape_corners[3 * c + 2]);
+                    p.x *= sc[0]; p.y *= sc[1]; p.z *= sc[2];
+                    corners.push_back(p);
+                }
+                hw.hulls.push_back(build_hull(corners));
+                it = inst.emplace(key, (int32_t)hw.hulls.size() - 1).first;
+            }
+            const HostHull& h = hw.hulls[it->second];
+            if (!h.ok) hw.scene_err[s] |= ERR_HULL;
+            const double* P = d->body_position + 3 * g;
+            const double* R = d->body_rotation + 4 * g;
+            const double* LV = d->body_linear_velocity + 3 * g;
+            const double* AV = d->body_angular_velocity + 3 * g;
+            Q4 q = {R[0], R[1], R[2], R[3]};
+            hw.body.push_back(make_body(h, v3(P[0], P[1], P[2]), q, v3(LV[0], LV[1], LV[2]), v3(AV[0], AV[1], AV[2]),
+                                        d->body_mass[g], d->body_fixed[g] != 0, d->body_material + 3 * g,
+                                        hw.scene_err[s]));
+            hw.body_hull.push_back(it->second);
+        }
+        hw.scene_body_off.push_back((int32_t)hw.body.size());
+        if (d->scene_constraint_offset && d->constraints) {
+            uint32_t b0 = d->scene_body_offset[s], n = d->scene_body_offset[s + 1] - b0;
+            for (uint32_t k = d->scene_constraint_offset[s]; k < d->scene_constraint_offset[s + 1]; ++k) {
+                const rp_constraint_desc& cd = d->constraints[k];
+                if (cd.body1 >= n || cd.body2 >= n || cd.type == RP_COLLISION_CONSTRAINT || cd.type < 0 || cd.type > 4) {
+                    set_msg("invalid external constraint");
+                    return false;
+                }
+                ExtC c;
+                memset(&c, 0, sizeof(c));
+                c.type = cd.type;
+                c.b1 = (int32_t)(b0 + cd.body1);
+                c.b2 = (int32_t)(b0 + cd.body2);
+                c.limited = cd.limited;
+                for (int a = 0; a < 4; ++a) { c.axis[a] = cd.axis[a]; c.limits[a] = cd.limits[a]; }
+                c.r1 = v3(cd.r1_lc[0], cd.r1_lc[1], cd.r1_lc[2]);
+                c.r2 = v3(cd.r2_lc[0], cd.r2_lc[1], cd.r2_lc[2]);
+                c.distance = v3(cd.distance[0], cd.distance[1], cd.distance[2]);
+                c.compliance = cd.compliance;
+                hw.ext.push_back(c);
+            }
+        }
+        hw.scene_ext_off.push_back((int32_t)hw.ext.size());
+    }
+    (void)nb;
+    return true;
+}
+
+}  // namespace rp
+
+using namespace rp;
+
+struct rp_world {
+    HostWorld hw;
+    DeviceImage im;
+    hipStream_t stream = nullptr;
+    std::vector<Body> host_bodies;  // scratch for state I/O
+};
+
+static bool world_init(rp_world* w) {
+    if (hipStreamCreate(&w->stream) != hipSuccess) {
+        set_msg("hipStreamCreate failed");
+        return false;
+    }
+    return build_image(w->hw, w->im);
+}
+
+static void world_free(rp_world* w) {
+    if (!w) return;
+    if (w->stream) (void)hipStreamSynchronize(w->stream);
+    w->im.release();
+    if (w->stream) (void)hipStreamDestroy(w->stream);
+    delete w;
+}
+
+static uint32_t world_run(rp_world* w, const FrameParams& fp, uint32_t frames, bool sync) {
+    for (uint32_t f = 0; f < frames; ++f) {
+        hipError_t e = launch_frame(w->im.d, fp, w->stream);
+        if (e != hipSuccess) {
+            set_msg(std::string("kernel launch: ") + hipGetErrorString(e));
+            return RP_ERR_RUNTIME;
+        }
+    }
+    if (sync) {
+        hipError_t e = hipStreamSynchronize(w->stream);
+        if (e != hipSuccess) {
+            set_msg(std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
+            return RP_ERR_RUNTIME;
+        }
+    }
+    return 0;
+}
+
+static bool download_bodies(rp_world* w) {
+    w->host_bodies.resize(w->hw.body.size());
+    if (w->host_bodies.empty()) return true;
+    RP_CHECK(hipStreamSynchronize(w->stream));
+    RP_CHECK(hipMemcpy(w->host_bodies.data(), w->im.d.body, w->host_bodies.size() * sizeof(Body), hipMemcpyDeviceToHost));
+    return true;
+}
+
+static bool upload_bodies(rp_world* w) {
+    if (w->host_bodies.empty()) return true;
+    RP_CHECK(hipMemcpy(w->im.d.body, w->host_bodies.data(), w->host_bodies.size() * sizeof(Body), hipMemcpyHostToDevice));
+    return true;
+}
+
+extern "C" {
+
+rp_world* rp_world_create(const rp_world_desc* desc) {
+    if (!desc) { set_msg("null descriptor"); return nullptr; }
+    rp_world* w = new rp_world();
+    if (!host_world_from_desc(desc, w->hw) || !world_init(w)) {
+        world_free(w);
+        return nullptr;
+    }
+    return w;
+}
+
+void rp_world_destroy(rp_world* w) { world_free(w); }
+uint32_t rp_world_num_bodies(const rp_world* w) { return (uint32_t)w->hw.body.size(); }
+uint32_t rp_world_num_scenes(const rp_world* w) { return (uint32_t)w->hw.scene_body_off.size() - 1; }
+void* rp_world_stream(rp_world* w) { return (void*)w->stream; }
+
+static uint32_t step_impl(rp_world* w, double dt, uint32_t frames, uint32_t substeps, uint32_t iters, int32_t coll,
+                          double gravity, bool sync) {
+    if (!(dt > 0.0)) return 0;  // pbd.cpp:471 (dt <= 0 is a no-op; NaN too)
+    FrameParams fp;
+    fp.dt = dt;
+    fp.h = dt / substeps;
+    fp.gravity = gravity;
+    fp.substeps = substeps;
+    fp.iters = iters;
+    fp.collisions = coll != 0;
+    return world_run(w, fp, frames, sync);
+}
+
+uint32_t rp_world_step(rp_world* w, double dt, uint32_t frames, uint32_t substeps, uint32_t iters, int32_t coll,
+                       double gravity) {
+    return step_impl(w, dt, frames, substeps, iters, coll, gravity, true);
+}
+uint32_t rp_world_step_async(rp_world* w, double dt, uint32_t frames, uint32_t substeps, uint32_t iters, int32_t coll,
+                             double gravity) {
+    return step_impl(w, dt, frames, substeps, iters, coll, gravity, false);
+}
+uint32_t rp_world_synchronize(rp_world* w) {
+    hipError_t e = hipStreamSynchronize(w->stream);
+    if (e != hipSuccess) { set_msg(hipGetErrorString(e)); return RP_ERR_RUNTIME; }
+    return 0;
+}
+
+static void put3(double* p, size_t i, V3 v) { if (p) { p[3 * i] = v.x; p[3 * i + 1] = v.y; p[3 * i + 2] = v.z; } }
+static void put4(double* p, size_t i, Q4 q) { if (p) { p[4 * i] = q.x; p[4 * i + 1] = q.y; p[4 * i + 2] = q.z; p[4 * i + 3] = q.w; } }
+static V3 get3(const double* p, size_t i) { return v3(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
+static Q4 get4(const double* p, size_t i) { Q4 q = {p[4 * i], p[4 * i + 1], p[4 * i + 2], p[4 * i + 3]}; return q; }
+
+void rp_world_get_state(rp_world* w, const rp_state* o) {
+    if (!download_bodies(w)) return;
+    for (size_t i = 0; i < w->host_bodies.size(); ++i) {
+        const Body& e = w->host_bodies[i];
+        put3(o->position, i, e.x);
+        put4(o->rotation, i, e.q);
+        put3(o->linear_velocity, i, e.v);
+        put3(o->angular_velocity, i, e.w);
+        put3(o->previous_position, i, e.px);
+        put4(o->previous_rotation, i, e.pq);
+        put3(o->previous_linear_velocity, i, e.pv);
+        put3(o->previous_angular_velocity, i, e.pw);
+        if (o->active) o->active[i] = e.active;
+        if (o->deactivation_time) o->deactivation_time[i] = e.deact;
+    }
+}
+
+void rp_world_set_state(rp_world* w, const rp_state* in) {
+    if (!download_bodies(w)) return;
+    for (size_t i = 0; i < w->host_bodies.size(); ++i) {
+        Body& e = w->host_bodies[i];
+        if (in->position) e.x = get3(in->position, i);
+        if (in->rotation) e.q = get4(in->rotation, i);
+        if (in->linear_velocity) e.v = get3(in->linear_velocity, i);
+        if (in->angular_velocity) e.w = get3(in->angular_velocity, i);
+        if (in->previous_position) e.px = get3(in->previous_position, i);
+        if (in->previous_rotation) e.pq = get4(in->previous_rotation, i);
+        if (in->previous_linear_velocity) e.pv = get3(in->previous_linear_velocity, i);
+        if (in->previous_angular_velocity) e.pw = get3(in->previous_angular_velocity, i);
+        if (in->active) e.active = in->active[i];
+        if (in->deactivation_time) e.deact = in->deactivation_time[i];
+    }
+    upload_bodies(w);
+}
+
+uint32_t rp_world_error(rp_world* w, uint32_t scene) {
+    if (scene >= rp_world_num_scenes(w)) return 0;
+    uint32_t e = 0;
+    if (hipStreamSynchronize(w->stream) != hipSuccess ||
+        hipMemcpy(&e, w->im.d.scene_err + scene, 4, hipMemcpyDeviceToHost) != hipSuccess)
+        return RP_ERR_RUNTIME;
+    return e;
+}
+
+uint64_t rp_world_epa_failures(rp_world* w) {
+    unsigned long long v = 0;
+    if (hipStreamSynchronize(w->stream) != hipSuccess) return 0;
+    if (hipMemcpy(&v, w->im.d.stats + STAT_EPA_FAIL, 8, hipMemcpyDeviceToHost) != hipSuccess) return 0;
+    return v;
+}
+
+void rp_world_body_info(rp_world* w, double* inv_mass, double* inertia, double* inv_inertia, double* radius) {
+    for (size_t i = 0; i < w->hw.body.size(); ++i) {
+        const Body& e = w->hw.body[i];
+        if (inv_mass) inv_mass[i] = e.inv_mass;
+        if (inertia) memcpy(inertia + 9 * i, e.I.m, 72);
+        if (inv_inertia) memcpy(inv_inertia + 9 * i, e.Iinv.m, 72);
+        if (radius) radius[i] = e.radius;
+    }
+}
+
+uint32_t rp_world_broad_pairs(rp_world* w, uint32_t scene, uint32_t* out, uint32_t cap) {
+    const DevWorld& d = w->im.d;
+    if ((int)scene >= d.ns) return 0;
+    if (launch_broad(d, w->stream) != hipSuccess || hipStreamSynchronize(w->stream) != hipSuccess) return 0;
+    int32_t n = 0, base = 0;
+    (void)hipMemcpy(&n, d.pair_count + scene, 4, hipMemcpyDeviceToHost);
+    (void)hipMemcpy(&base, d.scene_pair_base + scene, 4, hipMemcpyDeviceToHost);
+    std::vector<int2> pr(std::max(n, 1));
+    if (n > 0) (void)hipMemcpy(pr.data(), d.pairs + base, n * sizeof(int2), hipMemcpyDeviceToHost);
+    for (int32_t k = 0; k < n && (uint32_t)k < cap; ++k) {
+        out[2 * k] = (uint32_t)pr[k].x;
+        out[2 * k + 1] = (uint32_t)pr[k].y;
+    }
+    return (uint32_t)n;
+}
+
+uint32_t rp_world_contacts(rp_world* w, uint32_t scene, uint32_t i, uint32_t j, double* out, uint32_t cap) {
+    const DevWorld& d = w->im.d;
+    if ((int)scene >= d.ns) return 0;
+    int b0 = w->hw.scene_body_off[scene], n = w->hw.scene_body_off[scene + 1] - b0;
+    if ((int)i >= n || (int)j >= n) return 0;
+    double* dout = nullptr;
+    int* dcount = nullptr;
+    uint32_t* derr = nullptr;
+    uint32_t nc = 0;
+    if (hipMalloc(&dout, 9 * MAXC * sizeof(double)) == hipSuccess && hipMalloc(&dcount, 4) == hipSuccess &&
+        hipMalloc(&derr, 4) == hipSuccess &&
+        launch_pair_contacts(d, b0 + (int)i, b0 + (int)j, dout, dcount, derr, w->stream) == hipSuccess &&
+        hipStreamSynchronize(w->stream) == hipSuccess) {
+        int c = 0;
+        uint32_t e = 0;
+        double buf[9 * MAXC];
+        (void)hipMemcpy(&c, dcount, 4, hipMemcpyDeviceToHost);
+        (void)hipMemcpy(&e, derr, 4, hipMemcpyDeviceToHost);
+        (void)hipMemcpy(buf, dout, sizeof(buf), hipMemcpyDeviceToHost);
+        nc = (uint32_t)c;
+        for (uint32_t k = 0; k < nc && k < cap; ++k) memcpy(out + 9 * k, buf + 9 * k, 72);
+        g_last_err = e;
+    }
+    if (dout) (void)hipFree(dout);
+    if (dcount) (void)hipFree(dcount);
+    if (derr) (void)hipFree(derr);
+    return nc;
+}
+
+void rp_world_hull(rp_world* w, uint32_t body, uint32_t counts[6], double* vertices, double* face_normals,
+                   uint32_t* face_off, uint32_t* face_el, uint32_t* vf_off, uint32_t* vf, uint32_t* vn_off,
+                   uint32_t* vn, uint32_t* ff_off, uint32_t* ff) {
+    memset(counts, 0, 6 * sizeof(uint32_t));
+    if (body >= w->hw.body.size()) return;
+    const HostHull& h = w->hw.hulls[w->hw.body_hull[body]];
+    counts[0] = (uint32_t)h.v.size(); counts[1] = (uint32_t)h.fn.size();
+    counts[2] = (uint32_t)h.fe.size(); counts[3] = (uint32_t)h.vf.size();
+    counts[4] = (uint32_t)h.vn.size(); counts[5] = (uint32_t)h.ff.size();
+    auto cp = [](uint32_t* dst, const std::vector<int32_t>& src) { if (dst) for (size_t i = 0; i < src.size(); ++i) dst[i] = (uint32_t)src[i]; };
+    if (vertices) memcpy(vertices, h.v.data(), h.v.size() * 24);
+    if (face_normals) memcpy(face_normals, h.fn.data(), h.fn.size() * 24);
+    cp(face_off, h.fe_off); cp(face_el, h.fe); cp(vf_off, h.vf_off); cp(vf, h.vf);
+    cp(vn_off, h.vn_off); cp(vn, h.vn); cp(ff_off, h.ff_off); cp(ff, h.ff);
+}
+
+uint32_t rp_fp64_selftest(const double* a, const double* b, double* out, uint64_t n) {
+    double *da = nullptr, *db = nullptr, *dout = nullptr;
+    uint32_t rc = RP_ERR_RUNTIME;
+    size_t bytes = std::max<uint64_t>(n, 1) * sizeof(double);
+    if (hipMalloc(&da, bytes) == hipSuccess && hipMalloc(&db, bytes) == hipSuccess &&
+        hipMalloc(&dout, 5 * bytes) == hipSuccess && hipMemcpy(da, a, n * 8, hipMemcpyHostToDevice) == hipSuccess &&
+        hipMemcpy(db, b, n * 8, hipMemcpyHostToDevice) == hipSuccess &&
+        launch_fp64_selftest(da, db, dout, (int64_t)n, nullptr) == hipSuccess && hipDeviceSynchronize() == hipSuccess &&
+        hipMemcpy(out, dout, 5 * n * 8, hipMemcpyDeviceToHost) == hipSuccess)
+        rc = 0;
+    if (da) (void)hipFree(da);
+    if (db) (void)hipFree(db);
+    if (dout) (void)hipFree(dout);
+    return rc;
+}
+
+uint32_t rp_last_error(void) { return g_last_err; }
+const char* rp_last_error_string(void) { return g_last_msg.c_str(); }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Drop-in step over the reference's Entity structs
+// ---------------------------------------------------------------------------
+static_assert(sizeof(rp_entity) == 504, "Entity layout (src/entity.h:17-47)");
+static_assert(offsetof(rp_entity, world_position) == 24, "Entity layout");
+static_assert(offsetof(rp_entity, world_rotation) == 48, "Entity layout");
+static_assert(offsetof(rp_entity, colliders) == 136, "Entity layout");
+static_assert(offsetof(rp_entity, inverse_mass) == 160, "Entity layout");
+static_assert(offsetof(rp_entity, angular_velocity) == 312, "Entity layout");
+static_assert(offsetof(rp_entity, fixed) == 360, "Entity layout");
+static_assert(offsetof(rp_entity, deactivation_time) == 368, "Entity layout");
+static_assert(offsetof(rp_entity, previous_world_position) == 400, "Entity layout");
+static_assert(offsetof(rp_entity, previous_angular_velocity) == 480, "Entity layout");
+static_assert(sizeof(rp_constraint) == 144, "Constraint layout (src/physics/pbd.h:79-91)");
+static_assert(sizeof(rp_collider) == 64, "Collider layout (src/physics/collider.h:38-44)");
+static_assert(sizeof(rp_physics_force) == 52, "Physics_Force layout (src/entity.h:11-15)");
+
+static size_t la_len(const void* p) { return p ? ((const size_t*)p)[-1] : 0; }  // light_array.h:87-89
+static V3 to_v3(const rp_vec3& v) { return v3(v.x, v.y, v.z); }
+static rp_vec3 from_v3(V3 v) { rp_vec3 r = {v.x, v.y, v.z}; return r; }
+
+// The drop-in keeps one device world for the last entity set it saw and only
+// re-uploads body records while the set (entities, colliders, constraint
+// endpoints) is unchanged.
+struct DropIn {
+    std::mutex mu;
+    rp_world* w = nullptr;
+    std::vector<const void*> key;
+};
+static DropIn g_dropin;
+
+static bool dropin_build(rp_entity** ents, size_t n, rp_constraint* cons, HostWorld& hw, uint32_t& err) {
+    hw.scene_err.assign(1, 0);
+    std::unordered_map<uint64_t, int32_t> id2idx;
+    for (size_t i = 0; i < n; ++i) id2idx[ents[i]->id] = (int32_t)i;
+    for (size_t i = 0; i < n; ++i) {
+        const rp_entity* e = ents[i];
+        if (la_len(e->colliders) != 1 || e->colliders[0].type != RP_COLLIDER_CONVEX_HULL) {
+            set_msg("drop-in supports entities with exactly one convex-hull collider");
+            return false;
+        }
+        const rp_convex_hull& ch = e->colliders[0].u.convex_hull;
+        HostHull h;
+        size_t nv = la_len(ch.vertices), nf = la_len(ch.faces);
+        for (size_t k = 0; k < nv; ++k) h.v.push_back(to_v3(ch.vertices[k]));
+        std::vector<std::vector<int32_t>> fe(nf), vf(nv), vn(nv), ff(nf);
+        for (size_t f = 0; f < nf; ++f) {
+            h.fn.push_back(to_v3(ch.faces[f].normal));
+            for (size_t k = 0; k < la_len(ch.faces[f].elements); ++k) fe[f].push_back((int32_t)ch.faces[f].elements[k]);
+            for (size_t k = 0; k < la_len(ch.face_to_neighbors[f]); ++k) ff[f].push_back((int32_t)ch.face_to_neighbors[f][k]);
+        }
+        for (size_t v = 0; v < nv; ++v) {
+            for (size_t k = 0; k < la_len(ch.vertex_to_faces[v]); ++k) vf[v].push_back((int32_t)ch.vertex_to_faces[v][k]);
+            for (size_t k = 0; k < la_len(ch.vertex_to_neighbors[v]); ++k) vn[v].push_back((int32_t)ch.vertex_to_neighbors[v][k]);
+        }
+        flatten(fe, h.fe_off, h.fe);
+        flatten(vf, h.vf_off, h.vf);
+        flatten(vn, h.vn_off, h.vn);
+        flatten(ff, h.ff_off, h.ff);
+        hw.hulls.push_back(h);
+        hw.body_hull.push_back((int32_t)i);
+        Body b;
+        memset(&b, 0, sizeof(b));
+        hw.body.push_back(b);
+    }
+    hw.scene_body_off.push_back((int32_t)n);
+    for (size_t k = 0; k < la_len(cons); ++k) {
+        const rp_constraint& c = cons[k];
+        auto i1 = id2idx.find(c.e1_id), i2 = id2idx.find(c.e2_id);
+        if (i1 == id2idx.end() || i2 == id2idx.end() || c.type == RP_COLLISION_CONSTRAINT || c.type < 0 || c.type > 4) {
+            set_msg("external constraint endpoint not in the entity array");
+            err |= ERR_SELECT;
+            return false;
+        }
+        ExtC x;
+        memset(&x, 0, sizeof(x));
+        x.type = c.type;
+        x.b1 = i1->second;
+        x.b2 = i2->second;
+        switch (c.type) {
+        case RP_POSITIONAL_CONSTRAINT:
+            x.r1 = to_v3(c.u.positional_constraint.r1_lc); x.r2 = to_v3(c.u.positional_constraint.r2_lc);
+            x.compliance = c.u.positional_constraint.compliance; x.distance = to_v3(c.u.positional_constraint.distance);
+            break;
+        case RP_MUTUAL_ORIENTATION_CONSTRAINT:
+            x.compliance = c.u.mutual_orientation_constraint.compliance;
+            break;
+        case RP_HINGE_JOINT_CONSTRAINT: {
+            const rp_hinge_joint_constraint& hj = c.u.hinge_joint_constraint;
+            x.r1 = to_v3(hj.r1_lc); x.r2 = to_v3(hj.r2_lc); x.compliance = hj.compliance;
+            x.axis[0] = hj.e1_aligned_axis; x.axis[1] = hj.e2_aligned_axis;
+            x.axis[2] = hj.e1_limit_axis; x.axis[3] = hj.e2_limit_axis;
+            x.limited = hj.limited; x.limits[0] = hj.lower_limit; x.limits[1] = hj.upper_limit;
+        } break;
+        case RP_SPHERICAL_JOINT_CONSTRAINT: {
+            const rp_spherical_joint_constraint& sj = c.u.spherical_joint_constraint;
+            x.r1 = to_v3(sj.r1_lc); x.r2 = to_v3(sj.r2_lc);
+            x.axis[0] = sj.e1_swing_axis; x.axis[1] = sj.e2_swing_axis;
+            x.axis[2] = sj.e1_twist_axis; x.axis[3] = sj.e2_twist_axis;
+            x.limits[0] = sj.swing_lower_limit; x.limits[1] = sj.swing_upper_limit;
+            x.limits[2] = sj.twist_lower_limit; x.limits[3] = sj.twist_upper_limit;
+        } break;
+        }
+        hw.ext.push_back(x);
+    }
+    hw.scene_ext_off.push_back((int32_t)hw.ext.size());
+    return true;
+}
+
+// Entity -> body record, with the frame's force / torque sums
+// (calculate_external_force/torque, physics_util.cpp:5-23).
+static Body entity_body(const rp_entity* e) {
+    Body b;
+    memset(&b, 0, sizeof(b));
+    b.x = to_v3(e->world_position);
+    b.q = {e->world_rotation.x, e->world_rotation.y, e->world_rotation.z, e->world_rotation.w};
+    b.v = to_v3(e->linear_velocity);
+    b.w = to_v3(e->angular_velocity);
+    b.px = to_v3(e->previous_world_position);
+    b.pq = {e->previous_world_rotation.x, e->previous_world_rotation.y, e->previous_world_rotation.z,
+            e->previous_world_rotation.w};
+    b.pv = to_v3(e->previous_linear_velocity);
+    b.pw = to_v3(e->previous_angular_velocity);
+    b.deact = e->deactivation_time;
+    b.inv_mass = e->inverse_mass;
+    memcpy(b.I.m, e->inertia_tensor.data, 72);
+    memcpy(b.Iinv.m, e->inverse_inertia_tensor.data, 72);
+    b.mu_s = e->static_friction_coefficient;
+    b.mu_d = e->dynamic_friction_coefficient;
+    b.e = e->restitution_coefficient;
+    b.radius = e->bounding_sphere_radius;
+    b.fixed = e->fixed;
+    b.active = e->active;
+    V3 F = v3(0.0, 0.0, 0.0), T = v3(0.0, 0.0, 0.0);
+    for (size_t i = 0; e->forces && i < la_len(e->forces); ++i) F = add(F, to_v3(e->forces[i].force));
+    for (size_t i = 0; e->forces && i < la_len(e->forces); ++i) {
+        V3 dist = sub(to_v3(e->forces[i].position), v3(0.0, 0.0, 0.0));
+        T = add(T, cross(dist, to_v3(e->forces[i].force)));
+    }
+    b.F = F;
+    b.T = T;
+    return b;
+}
+
+extern "C" uint32_t rp_pbd_simulate(double dt, rp_entity** ents, rp_constraint* cons, uint32_t substeps,
+                                    uint32_t iters, int32_t collisions) {
+    g_last_err = 0;
+    if (!(dt > 0.0) || !ents) return 0;  // pbd.cpp:471
+    size_t n = la_len(ents);
+    std::lock_guard<std::mutex> lock(g_dropin.mu);
+    std::vector<const void*> key;
+    for (size_t i = 0; i < n; ++i) {
+        key.push_back(ents[i]);
+        key.push_back(ents[i]->colliders);
+        key.push_back(ents[i]->colliders ? ents[i]->colliders[0].u.convex_hull.vertices : nullptr);
+    }
+    key.push_back((const void*)(uintptr_t)la_len(cons));
+    for (size_t k = 0; k < la_len(cons); ++k) {
+        key.push_back((const void*)(uintptr_t)cons[k].e1_id);
+        key.push_back((const void*)(uintptr_t)cons[k].e2_id);
+        key.push_back((const void*)(uintptr_t)cons[k].type);
+    }
+    uint32_t err = 0;
+    if (!g_dropin.w || key != g_dropin.key) {
+        if (g_dropin.w) world_free(g_dropin.w);
+        g_dropin.w = new rp_world();
+        g_dropin.key.clear();
+        if (!dropin_build(ents, n, cons, g_dropin.w->hw, err) || !world_init(g_dropin.w)) {
+            world_free(g_dropin.w);
+            g_dropin.w = nullptr;
+            g_last_err = err | RP_ERR_RUNTIME;
+            return g_last_err;
+        }
+        g_dropin.key = key;
+    }
+    rp_world* w = g_dropin.w;
+    // constraint payloads may change between calls (compliance, limits)
+    if (la_len(cons)) {
+        HostWorld tmp;
+        uint32_t e2 = 0;
+        if (dropin_build(ents, n, cons, tmp, e2) && !tmp.ext.empty())
+            (void)hipMemcpy((void*)w->im.d.ext, tmp.ext.data(), tmp.ext.size() * sizeof(ExtC), hipMemcpyHostToDevice);
+    }
+    w->host_bodies.resize(n);
+    for (size_t i = 0; i < n; ++i) w->host_bodies[i] = entity_body(ents[i]);
+    if (!upload_bodies(w)) return g_last_err = RP_ERR_RUNTIME;
+    (void)hipMemset(w->im.d.scene_err, 0, 4);
+    FrameParams fp;
+    fp.dt = dt;
+    fp.h = dt / substeps;
+    fp.gravity = 0.0;
+    fp.substeps = substeps;
+    fp.iters = iters;
+    fp.collisions = collisions != 0;
+    fp.forces_given = true;
+    if (world_run(w, fp, 1, true)) return g_last_err = RP_ERR_RUNTIME;
+    if (!download_bodies(w)) return g_last_err = RP_ERR_RUNTIME;
+    for (size_t i = 0; i < n; ++i) {
+        rp_entity* e = ents[i];
+        const Body& b = w->host_bodies[i];
+        e->world_position = from_v3(b.x);
+        e->world_rotation = {b.q.x, b.q.y, b.q.z, b.q.w};
+        e->linear_velocity = from_v3(b.v);
+        e->angular_velocity = from_v3(b.w);
+        e->previous_world_position = from_v3(b.px);
+        e->previous_world_rotation = {b.pq.x, b.pq.y, b.pq.z, b.pq.w};
+        e->previous_linear_velocity = from_v3(b.pv);
+        e->previous_angular_velocity = from_v3(b.pw);
+        e->active = b.active;
+        e->deactivation_time = b.deact;
+    }
+    g_last_err = rp_world_error(w, 0);
+    return g_last_err;
+}
+
+extern "C" void physics_simulate(rp_entity** entities, double dt) { rp_pbd_simulate(dt, entities, nullptr, 20, 1, 1); }
+
+// The reference's own C++ entry points (src/physics/pbd.h:93-94).
+struct Entity;
+struct Constraint;
+__attribute__((visibility("default"))) void pbd_simulate(double dt, Entity** entities, unsigned int num_substeps,
+                                                         unsigned int num_pos_iters, int enable_collisions) {
+    rp_pbd_simulate(dt, (rp_entity**)entities, nullptr, num_substeps, num_pos_iters, enable_collisions);
+}
+__attribute__((visibility("default"))) void pbd_simulate_with_constraints(double dt, Entity** entities,
+                                                                          Constraint* external_constraints,
+                                                                          unsigned int num_substeps,
+                                                                          unsigned int num_pos_iters,
+                                                                          int enable_collisions) {
+    rp_pbd_simulate(dt, (rp_entity**)entities, (rp_constraint*)external_constraints, num_substeps, num_pos_iters,
+                    enable_collisions);
+}

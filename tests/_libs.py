@@ -71,8 +71,13 @@ _cache = {}
 
 
 def load(kind: str):
+    """Library for `kind`; None if that library is not built here."""
     if kind in _cache:
         return _cache[kind]
+    if kind == "rp":
+        lib = pkg.library()  # raises if librawphys.so is missing: no fallback
+        _cache[kind] = lib
+        return lib
     path = {"orc": ORACLE_SO, "ref": REF_SO}[kind]
     if not path.exists():
         return None
