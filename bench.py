@@ -1,0 +1,251 @@
+#!/usr/bin/env python3
+"""Throughput of the raw-physics step on one MI355X.
+
+Workload (batched uniform): 4096 independent scenes, each a static 50x1x50
+floor plus 16 dynamic bodies (8 cubes with 8-vertex hulls, 8 icospheres with
+42-vertex hulls, size U(0.5, 1.0), mass = hull volume), dropped from a 4x4
+grid, dt = 1/60, 20 substeps, 1 position iteration, FP64, seeded synthetic
+inputs. One "step" is one frame of all 4096 scenes (20 substeps each).
+
+Metric: body-substeps/s = dynamic bodies x substeps x frames / second, for
+  * the batched method (all scenes resident in HBM, one launch sequence per
+    frame for the whole batch)            -> `value`
+  * the plain method (one scene per step call, state uploaded and written
+    back every frame, as the drop-in pbd_simulate does)  -> `plain`
+  * the reference CPU step on the host    -> `cpu_baseline`
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import importlib.util
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+METRIC = "body-substeps/s, 4096 scenes×16 bodies, batched vs plain; adjoint steps/s"
+
+
+def _package():
+    if "raw_physics_amd" in sys.modules:
+        return sys.modules["raw_physics_amd"]
+    pkg_dir = ROOT / "raw-physics_amd"
+    spec = importlib.util.spec_from_file_location("raw_physics_amd", pkg_dir / "__init__.py",
+                                                  submodule_search_locations=[str(pkg_dir)])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["raw_physics_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(scenes_mod, frames: int, budget_s: float):
+    """Times the reference step (oracle/_ref, compiled from the reference's own
+    sources) or, where that build is absent, the oracle's C restatement, on
+    one host core over a bounded sample of the same workload."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from _libs import SimWorld, load  # test infrastructure: the CPU checker only
+
+    kind = "ref" if load("ref") is not None else "orc"
+    # estimate cost on one scene, then size the sample to the time budget
+    probe = SimWorld(kind, scenes_mod.batched_uniform(1, first_seed=0))
+    t0 = time.perf_counter()
+    probe.step(min(frames, 60))
+    per_scene_frame = (time.perf_counter() - t0) / min(frames, 60)
+    probe.close()
+    n_scenes = max(1, min(64, int(budget_s / (per_scene_frame * frames))))
+    world = scenes_mod.batched_uniform(n_scenes, first_seed=0)
+    sim = SimWorld(kind, world)
+    t0 = time.perf_counter()
+    sim.step(frames)
+    dt = time.perf_counter() - t0
+    dyn = world.dynamic_bodies()
+    return {"value": dyn * 20 * frames / dt, "unit": "body-substeps/s", "cores": 1,
+            "kind": "reference" if kind == "ref" else "port",
+            "sample": f"{n_scenes} of the 4096 scenes (seeds 0..{n_scenes - 1}) x {frames} frames, "
+                      f"single thread, {dt:.1f} s"}
+
+
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (and matrix) peak
+
+
+def op_counts(scenes_mod, n_scenes: int, frames: int):
+    """Algorithmic FP64 work per phase on a sample of the workload, counted by
+    the oracle's op-counting build (oracle/libopcount.so: every add, subtract,
+    multiply, divide and square root the restatement executes)."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from _libs import SimWorld, _bind  # measurement infrastructure only
+
+    path = ROOT / "oracle" / "libopcount.so"
+    if not path.exists():
+        return None
+    lib = _bind(ctypes.CDLL(str(path)), "orc_")
+    lib.orc_opcount_get.argtypes = [ctypes.c_void_p]
+    world = scenes_mod.batched_uniform(n_scenes, first_seed=0)
+    sim = SimWorld("orc", world, lib=lib)
+    lib.orc_opcount_reset()
+    pairs = contacts = 0
+    for _ in range(frames):
+        sim.step(1)
+        pairs += lib.orc_world_last_narrow_pairs(sim.h)
+        contacts += lib.orc_world_last_contacts(sim.h)
+    ops = np.zeros(8, dtype=np.uint64)
+    lib.orc_opcount_get(ops.ctypes.data)
+    sim.close()
+    names = ("frame", "integrate", "transform", "narrow", "position", "velocity_update", "velocity_solve")
+    d = {n: int(ops[i]) for i, n in enumerate(names)}
+    d["solve"] = d["position"] + d["velocity_update"] + d["velocity_solve"]
+    return {"ops": d, "scenes": n_scenes, "frames": frames, "narrow_pairs": int(pairs), "contacts": int(contacts),
+            "body_substeps": world.dynamic_bodies() * 20 * frames}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=600, help="timed frames of the whole batch")
+    ap.add_argument("--warmup", type=int, default=5, help="untimed frames before the timed region")
+    ap.add_argument("--scenes", type=int, default=4096)
+    ap.add_argument("--plain-scenes", type=int, default=16)
+    ap.add_argument("--plain-frames", type=int, default=60)
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel timing run")
+    ap.add_argument("--op-scenes", type=int, default=8, help="scenes in the op-count sample")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on exactly one GPU: scenes are independent, so a multi-GPU run is the same "
+              "kernel on each GPU's share of the scenes; no figure at several GPUs is produced.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    pkg = _package()
+    sc = pkg.scenes
+    substeps, dt = 20, 1.0 / 60.0
+
+    t0 = time.perf_counter()
+    world = sc.batched_uniform(args.scenes)
+    log(f"built {args.scenes} scenes ({world.num_bodies} bodies) in {time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
+    gw = pkg.World(world)
+    log(f"uploaded world in {time.perf_counter() - t0:.1f} s")
+    init = gw.state()
+
+    # warmup (first launches, code object load), then restore the initial state
+    if args.warmup > 0:
+        gw.step(args.warmup)
+        gw.set_state(init)
+    gw.synchronize()
+
+    t0 = time.perf_counter()
+    gw.step(args.steps, dt=dt, substeps=substeps, sync=False)
+    gw.synchronize()
+    elapsed = time.perf_counter() - t0
+    dyn = world.dynamic_bodies()
+    value = dyn * substeps * args.steps / elapsed
+    errs = gw.errors()
+    log(f"batched: {args.steps} frames in {elapsed:.3f} s -> {value:.4g} body-substeps/s; "
+        f"error words nonzero in {int((errs != 0).sum())} scenes; EPA non-convergence {gw.epa_failures()}")
+
+    # Per-kernel durations: the same run again from the initial state, with HIP
+    # events recorded around every launch on the world's stream.
+    ktimes = None
+    if not args.no_profile:
+        gw.set_state(init)
+        gw.set_profiling(True)
+        gw.step(args.steps, dt=dt, substeps=substeps)
+        ktimes = gw.kernel_times()
+        gw.set_profiling(False)
+        log("kernel times (ms, launches): " + json.dumps(ktimes))
+    gw.close()
+
+    plain = None
+    if not args.no_plain:
+        ps = sc.batched_uniform(args.plain_scenes)
+        worlds = []
+        for s in range(args.plain_scenes):
+            w1 = sc.World(name="plain")
+            sc.drop_scene(w1, s)
+            worlds.append(pkg.World(w1))
+        for w1 in worlds:  # warm
+            w1.step(1)
+        t0 = time.perf_counter()
+        for w1 in worlds:
+            for _ in range(args.plain_frames):
+                w1.step(1, dt=dt, substeps=substeps)
+                w1.state()  # write-back to the host, as the drop-in does every call
+        pt = time.perf_counter() - t0
+        plain = ps.dynamic_bodies() * substeps * args.plain_frames / pt
+        log(f"plain: {args.plain_scenes} scenes x {args.plain_frames} frames one scene per call -> {plain:.4g}")
+        for w1 in worlds:
+            w1.close()
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(sc, args.steps, args.cpu_budget)
+        log(f"cpu: {cpu}")
+
+    roofline = None
+    kernels = None
+    if ktimes is not None:
+        oc = op_counts(sc, min(args.op_scenes, args.scenes), args.steps)
+        if oc is not None:
+            scale = args.scenes / oc["scenes"]
+            kernels = {}
+            for cls in ("integrate", "transform", "narrow", "solve"):
+                ms, n = ktimes[cls]
+                flops = oc["ops"][cls] * scale
+                kernels[cls] = {"ms_total": ms, "launches": n, "avg_us": 1e3 * ms / max(n, 1),
+                                "algorithmic_flops": flops,
+                                "tflops": flops / (ms * 1e-3) / 1e12 if ms > 0 else None}
+            nar = kernels["narrow"]
+            roofline = {"bound": "fp64-valu", "kernel": "k_narrow (GJK + EPA + contact manifold)",
+                        "achieved": nar["tflops"], "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": nar["tflops"] / FP64_PEAK_TFLOPS if nar["tflops"] else None,
+                        "traffic": None,
+                        "flops_per_launch": nar["algorithmic_flops"] / max(nar["launches"], 1),
+                        "avg_launch_us": nar["avg_us"],
+                        "op_sample": f"{oc['scenes']} scenes x {oc['frames']} frames counted by the oracle's "
+                                     f"op-counting build, scaled x{scale:g} to {args.scenes} scenes; "
+                                     f"{oc['ops']['narrow'] / max(oc['narrow_pairs'], 1):.0f} FP64 ops per "
+                                     f"narrow-phase pair"}
+            log("roofline: " + json.dumps(roofline))
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "body-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded scenes)",
+        "config": {"workload": "batched uniform: 4096 scenes x (floor + 8 cubes + 8 ico42), 600 frames, "
+                               "dt=1/60, 20 substeps, 1 position iteration",
+                   "scenes": args.scenes, "dynamic_bodies": dyn, "frames": args.steps, "substeps": substeps},
+        "plain": {"value": plain, "unit": "body-substeps/s",
+                  "ratio_batched_over_plain": (value / plain) if plain else None},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -246,6 +246,15 @@ uint64_t rp_world_epa_failures(rp_world* w);
 /* The HIP stream the world's kernels run on (a hipStream_t). */
 void* rp_world_stream(rp_world* w);
 
+/* Profiling mode: HIP events around every kernel launch of the step, read
+ * after each frame (the frame loop then synchronizes once per frame).
+ * rp_world_kernel_times fills ms[8] / launches[8] indexed by kernel class:
+ * 0 frame kernels (forces, broad phase, islands, live-pair lists),
+ * 1 integrate, 2 hull transform, 3 narrow phase, 4 solve. Enabling resets
+ * the totals. */
+void rp_world_set_profiling(rp_world* w, int32_t on);
+void rp_world_kernel_times(rp_world* w, double* ms, uint64_t* launches);
+
 /* Inspection (used by parity tests): derived body data, broad-phase pairs
  * and contacts at the current poses, hull topology of a body. Same
  * semantics as the reference functions cited in the oracle's header. */

@@ -15,6 +15,17 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* The arithmetic type. It is double; the op-counting build (opcount.cpp)
+ * compiles this file as C++ with `real` a double wrapper that counts every
+ * add, subtract, multiply, divide and square root, attributed to the phase
+ * named by ORC_PHASE. RD() reads the double value of a real. */
+#ifndef ORC_COUNT_OPS
+typedef double real;
+#define RD(x) (x)
+#define ORC_PHASE(p) ((void)0)
+#endif
+enum { PH_FRAME = 0, PH_INTEGRATE, PH_TRANSFORM, PH_NARROW, PH_POSITION, PH_VELOCITY_UPDATE, PH_VELOCITY_SOLVE, PH_SETUP };
+
 /* ------------------------------------------------------------------ */
 /* Growable arrays with the reference's swap-remove semantics          */
 /* (include/light_array.h:116-151).                                    */
@@ -35,36 +46,36 @@
 /* ------------------------------------------------------------------ */
 /* Vector / quaternion / matrix math (include/gm.h, src/quaternion.cpp) */
 /* ------------------------------------------------------------------ */
-typedef struct { double x, y, z; } v3;
-typedef struct { double x, y, z, w; } qt;
-typedef struct { double m[3][3]; } m3;
-typedef struct { double m[4][4]; } m4;
+typedef struct { real x, y, z; } v3;
+typedef struct { real x, y, z, w; } qt;
+typedef struct { real m[3][3]; } m3;
+typedef struct { real m[4][4]; } m4;
 typedef struct { int32_t x, y, z; } i3;
 typedef struct { int32_t x, y; } i2;
 typedef ARR(i3) i3arr;
 typedef ARR(i2) i2arr;
 typedef ARR(v3) v3arr;
-typedef ARR(double) f64arr;
+typedef ARR(real) f64arr;
 
-static v3 V(double x, double y, double z) { v3 r; r.x = x; r.y = y; r.z = z; return r; }
+static v3 V(real x, real y, real z) { v3 r; r.x = x; r.y = y; r.z = z; return r; }
 /* gm.h:667-670 */
 static v3 add(v3 a, v3 b) { return V(a.x + b.x, a.y + b.y, a.z + b.z); }
 /* gm.h:679-682 */
 static v3 sub(v3 a, v3 b) { return V(a.x - b.x, a.y - b.y, a.z - b.z); }
 /* gm.h:615-618 */
-static v3 scl(double s, v3 v) { return V(s * v.x, s * v.y, s * v.z); }
+static v3 scl(real s, v3 v) { return V(s * v.x, s * v.y, s * v.z); }
 /* gm.h:691-694 */
-static double dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static real dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 /* gm.h:722-729 */
 static v3 cross(v3 a, v3 b) {
     return V(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
 /* gm.h:649-652 */
-static double len(v3 v) { return sqrt(v.x * v.x + v.y * v.y + v.z * v.z); }
+static real len(v3 v) { return sqrt(v.x * v.x + v.y * v.y + v.z * v.z); }
 /* gm.h:627-634 */
 static v3 nrm(v3 v) {
     if (!(v.x != 0.0 || v.y != 0.0 || v.z != 0.0)) return V(0.0, 0.0, 0.0);
-    double l = len(v);
+    real l = len(v);
     return V(v.x / l, v.y / l, v.z / l);
 }
 /* gm_vec3_invert is 0 - v, not -v (gm.h:588-590) */
@@ -95,12 +106,12 @@ static m3 m3_t(const m3* m) {
 }
 /* gm.h:347-368 */
 static int m3_inverse(const m3* m, m3* out) {
-    const double (*a)[3] = m->m;
-    double det = a[0][0] * (a[1][1] * a[2][2] - a[2][1] * a[1][2]) -
+    const real (*a)[3] = m->m;
+    real det = a[0][0] * (a[1][1] * a[2][2] - a[2][1] * a[1][2]) -
                  a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
                  a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
     if (det == 0.0) return 0;
-    double invdet = 1 / det;
+    real invdet = 1 / det;
     out->m[0][0] = (a[1][1] * a[2][2] - a[2][1] * a[1][2]) * invdet;
     out->m[0][1] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) * invdet;
     out->m[0][2] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) * invdet;
@@ -133,7 +144,7 @@ static qt q_mul(qt a, qt b) {
 }
 /* quaternion.cpp:144-147 */
 static qt q_nrm(qt q) {
-    double l = sqrt(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
+    real l = sqrt(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
     qt r;
     r.x = q.x / l; r.y = q.y / l; r.z = q.z / l; r.w = q.w / l;
     return r;
@@ -146,10 +157,10 @@ static qt q_inv(qt q) {
 }
 /* quaternion.cpp:257-268 */
 static v3 q_rot(qt q, v3 v) {
-    double ix = q.w * v.x + q.y * v.z - q.z * v.y;
-    double iy = q.w * v.y + q.z * v.x - q.x * v.z;
-    double iz = q.w * v.z + q.x * v.y - q.y * v.x;
-    double iw = -q.x * v.x - q.y * v.y - q.z * v.z;
+    real ix = q.w * v.x + q.y * v.z - q.z * v.y;
+    real iy = q.w * v.y + q.z * v.x - q.x * v.z;
+    real iz = q.w * v.z + q.x * v.y - q.y * v.x;
+    real iw = -q.x * v.x - q.y * v.y - q.z * v.z;
     return V((ix * q.w) + (iw * -q.x) + (iy * -q.z) - (iz * -q.y),
              (iy * q.w) + (iw * -q.y) + (iz * -q.x) - (ix * -q.z),
              (iz * q.w) + (iw * -q.z) + (ix * -q.y) - (iy * -q.x));
@@ -181,9 +192,9 @@ static m4 q_mat4(qt q) {
     return r;
 }
 /* quaternion.cpp:3-16 */
-static qt q_from_axis_radians(v3 axis, double angle) {
+static qt q_from_axis_radians(v3 axis, real angle) {
     if (len(axis) != 0.0) axis = nrm(axis);
-    double sang = sin(angle / 2.0);
+    real sang = sin(angle / 2.0);
     qt q;
     q.w = cos(angle / 2.0);
     q.x = axis.x * sang;
@@ -242,11 +253,11 @@ typedef struct {
     v3 x; qt q; v3 v; v3 w;      /* world_position, world_rotation, linear/angular velocity */
     v3 px; qt pq; v3 pv; v3 pw;  /* previous_* */
     int32_t fixed, active;
-    double deact;
-    double inv_mass;
+    real deact;
+    real inv_mass;
     m3 inertia, inv_inertia;
-    double radius;
-    double mu_s, mu_d, e;
+    real radius;
+    real mu_s, mu_d, e;
     v3 force, torque;            /* external force / torque of the current frame */
     hull h;
 } body;
@@ -256,13 +267,13 @@ typedef struct {
     uint32_t b1, b2;
     /* collision payload (pbd.h:30-36) */
     v3 r1, r2, n;
-    double lambda_t, lambda_n;
+    real lambda_t, lambda_n;
     /* joint payloads */
-    double compliance, lambda, lambda2, lambda3;
+    real compliance, lambda, lambda2, lambda3;
     v3 distance;
     int32_t axis[4];
     int32_t limited;
-    double limits[4];
+    real limits[4];
 } constraint;
 
 typedef struct {
@@ -301,7 +312,7 @@ static void collect_planar(const v3* hv, const i3* tris, u32arr* tri_nb, int* pr
     v3 v1 = hv[t.x], v2 = hv[t.y], v3_ = hv[t.z];
     v3 n = nrm(cross(sub(v2, v1), sub(v3_, v1)));
     if (processed[f]) return;
-    double projection = dot(n, target);
+    real projection = dot(n, target);
     if ((projection - 1.0) > -EPSILON && (projection - 1.0) < EPSILON) {
         ARR_PUSH(*out, t);
         processed[f] = 1;
@@ -474,11 +485,11 @@ static void hull_transform(hull* h, v3 t, qt q) {
     for (uint32_t i = 0; i < h->nv; ++i) {
         v3 v = h->v[i];
         /* gm_mat4_multiply_vec4 (gm.h:405-413) with w = 1 */
-        double x = v.x * M.m[0][0] + v.y * M.m[0][1] + v.z * M.m[0][2] + 1.0 * M.m[0][3];
-        double y = v.x * M.m[1][0] + v.y * M.m[1][1] + v.z * M.m[1][2] + 1.0 * M.m[1][3];
-        double z = v.x * M.m[2][0] + v.y * M.m[2][1] + v.z * M.m[2][2] + 1.0 * M.m[2][3];
-        double w = v.x * M.m[3][0] + v.y * M.m[3][1] + v.z * M.m[3][2] + 1.0 * M.m[3][3];
-        double s = 1.0 / w;
+        real x = v.x * M.m[0][0] + v.y * M.m[0][1] + v.z * M.m[0][2] + 1.0 * M.m[0][3];
+        real y = v.x * M.m[1][0] + v.y * M.m[1][1] + v.z * M.m[1][2] + 1.0 * M.m[1][3];
+        real z = v.x * M.m[2][0] + v.y * M.m[2][1] + v.z * M.m[2][2] + 1.0 * M.m[2][3];
+        real w = v.x * M.m[3][0] + v.y * M.m[3][1] + v.z * M.m[3][2] + 1.0 * M.m[3][3];
+        real s = 1.0 / w;
         h->tv[i] = V(s * x, s * y, s * z);
     }
     for (uint32_t i = 0; i < h->nf; ++i) {
@@ -496,9 +507,9 @@ static void hull_transform(hull* h, v3 t, qt q) {
 /* ------------------------------------------------------------------ */
 static uint32_t support_index(const hull* h, v3 d, uint32_t* err) {
     uint32_t sel = UINT32_MAX;
-    double max_dot = -DBL_MAX;
+    real max_dot = -DBL_MAX;
     for (uint32_t i = 0; i < h->nv; ++i) {
-        double dd = dot(h->tv[i], d);
+        real dd = dot(h->tv[i], d);
         if (dd > max_dot) { sel = i; max_dot = dd; }
     }
     if (sel == UINT32_MAX) { *err |= ORC_ERR_SUPPORT_NAN; sel = 0; }
@@ -653,19 +664,19 @@ static int gjk(const hull* h1, const hull* h2, simplex* out, uint32_t* err) {
 /* EPA (epa.cpp:8-237)                                                  */
 /* ------------------------------------------------------------------ */
 /* get_face_normal_and_distance_to_origin (epa.cpp:32-77) */
-static int face_normal(i3 f, const v3* poly, uint32_t np, v3* n_out, double* d_out) {
+static int face_normal(i3 f, const v3* poly, uint32_t np, v3* n_out, real* d_out) {
     v3 a = poly[f.x], b = poly[f.y], c = poly[f.z];
     v3 n = nrm(cross(sub(b, a), sub(c, a)));
     if (!(n.x != 0.0 || n.y != 0.0 || n.z != 0.0)) return 0;
     const double TOL = 0.0000000000000;
-    double d = dot(n, a);
+    real d = dot(n, a);
     if (d < -TOL) {
         n = inv3(n);
         d = -d;
     } else if (d >= -TOL && d <= TOL) {
         int found = 0;
         for (uint32_t i = 0; i < np; ++i) {
-            double ad = dot(n, poly[i]);
+            real ad = dot(n, poly[i]);
             if (ad < -TOL || ad > TOL) {
                 n = ad < -TOL ? n : inv3(n);
                 found = 1;
@@ -696,7 +707,7 @@ static void add_edge(i2arr* edges, i2 e, const v3* poly) {
 static v3 centroid(v3 p1, v3 p2, v3 p3) { return scl(1.0 / 3.0, add(add(p2, p3), p1)); }
 
 /* epa (epa.cpp:118-237). Returns 1 on convergence. */
-static int epa(const hull* h1, const hull* h2, const simplex* s, v3* n_out, double* pen_out, uint32_t* err) {
+static int epa(const hull* h1, const hull* h2, const simplex* s, v3* n_out, real* pen_out, uint32_t* err) {
     v3arr poly = {0};
     i3arr faces = {0};
     v3arr normals = {0};
@@ -706,10 +717,10 @@ static int epa(const hull* h1, const hull* h2, const simplex* s, v3* n_out, doub
     i3 f0 = {0, 1, 2}, f1 = {0, 2, 3}, f2 = {0, 3, 1}, f3 = {1, 2, 3};
     ARR_PUSH(faces, f0); ARR_PUSH(faces, f1); ARR_PUSH(faces, f2); ARR_PUSH(faces, f3);
     v3 min_n = V(0.0, 0.0, 0.0);
-    double min_d = DBL_MAX;
+    real min_d = DBL_MAX;
     int converged = 0, bad = 0;
     for (uint32_t i = 0; i < faces.n; ++i) {
-        v3 n; double d;
+        v3 n; real d;
         if (!face_normal(faces.d[i], poly.d, poly.n, &n, &d)) { bad = 1; goto done; }
         ARR_PUSH(normals, n);
         ARR_PUSH(dist, d);
@@ -717,7 +728,7 @@ static int epa(const hull* h1, const hull* h2, const simplex* s, v3* n_out, doub
     }
     for (uint32_t it = 0; it < 100; ++it) {
         v3 sp = support_md(h1, h2, min_n, err);
-        double d = dot(min_n, sp);
+        real d = dot(min_n, sp);
         if (fabs(d - min_d) < 0.0001) {
             *n_out = min_n;
             *pen_out = min_d;
@@ -744,7 +755,7 @@ static int epa(const hull* h1, const hull* h2, const simplex* s, v3* n_out, doub
         for (uint32_t i = 0; i < edges.n; ++i) {
             i3 nf = {edges.d[i].x, edges.d[i].y, (int32_t)new_idx};
             ARR_PUSH(faces, nf);
-            v3 n; double fd;
+            v3 n; real fd;
             if (!face_normal(nf, poly.d, poly.n, &n, &fd)) { bad = 1; goto done; }
             ARR_PUSH(normals, n);
             ARR_PUSH(dist, fd);
@@ -768,7 +779,7 @@ typedef struct { v3 normal, point; } plane;
 
 /* is_point_in_plane (clipping.cpp:12-19): note the float truncation */
 static int point_in_plane(const plane* p, v3 pos) {
-    float distance = -dot(p->normal, p->point);
+    float distance = (float)RD(-dot(p->normal, p->point));
     if (dot(pos, p->normal) + distance < 0.0) return 0;
     return 1;
 }
@@ -777,11 +788,11 @@ static int point_in_plane(const plane* p, v3 pos) {
 static int plane_edge(const plane* p, v3 start, v3 end, v3* out) {
     const double EPSILON = 0.000001;
     v3 ab = sub(end, start);
-    float ab_p = dot(p->normal, ab);
+    float ab_p = (float)RD(dot(p->normal, ab));
     if (fabs(ab_p) > EPSILON) {
-        float distance = -dot(p->normal, p->point);
+        float distance = (float)RD(-dot(p->normal, p->point));
         v3 p_co = scl(-distance, p->normal);
-        float fac = -dot(p->normal, sub(start, p_co)) / ab_p;
+        float fac = (float)RD(-dot(p->normal, sub(start, p_co)) / ab_p);
         fac = RP_MIN(RP_MAX(fac, 0.0), 1.0);
         *out = add(start, scl(fac, ab));
         return 1;
@@ -826,16 +837,16 @@ static void clip_polygon(const v3arr* in_poly, uint32_t nplanes, const plane* pl
 
 /* get_closest_point_polygon (clipping.cpp:115-119) */
 static v3 closest_on_plane(v3 pos, const plane* rp) {
-    double d = dot(scl(-1.0, rp->normal), rp->point);
+    real d = dot(scl(-1.0, rp->normal), rp->point);
     return sub(pos, scl(dot(rp->normal, pos) + d, rp->normal));
 }
 
 /* get_face_with_most_fitting_normal (clipping.cpp:136-152) */
 static uint32_t best_face(uint32_t sidx, const hull* h, v3 n, uint32_t* err) {
-    double max_proj = -DBL_MAX;
+    real max_proj = -DBL_MAX;
     uint32_t sel = UINT32_MAX;
     for (uint32_t i = h->vf_off[sidx]; i < h->vf_off[sidx + 1]; ++i) {
-        double proj = dot(h->tn[h->vf[i]], n);
+        real proj = dot(h->tn[h->vf[i]], n);
         if (proj > max_proj) { max_proj = proj; sel = h->vf[i]; }
     }
     if (sel == UINT32_MAX) { *err |= ORC_ERR_SELECT; sel = 0; }
@@ -846,7 +857,7 @@ static uint32_t best_face(uint32_t sidx, const hull* h, v3 n, uint32_t* err) {
 static void best_edge(uint32_t s1, uint32_t s2, const hull* h1, const hull* h2, v3 n, v3* edge_n, uint32_t sel[4],
                       uint32_t* err) {
     v3 sp1 = h1->tv[s1], sp2 = h2->tv[s2];
-    double max_dot = -DBL_MAX;
+    real max_dot = -DBL_MAX;
     int found = 0;
     for (uint32_t i = h1->vn_off[s1]; i < h1->vn_off[s1 + 1]; ++i) {
         v3 e1 = sub(sp1, h1->tv[h1->vn[i]]);
@@ -854,7 +865,7 @@ static void best_edge(uint32_t s1, uint32_t s2, const hull* h1, const hull* h2, 
             v3 e2 = sub(sp2, h2->tv[h2->vn[j]]);
             v3 cn = nrm(cross(e1, e2));
             v3 cni = inv3(cn);
-            double dd = dot(cn, n);
+            real dd = dot(cn, n);
             if (dd > max_dot) {
                 max_dot = dd; sel[0] = s1; sel[1] = h1->vn[i]; sel[2] = s2; sel[3] = h2->vn[j];
                 *edge_n = cn; found = 1;
@@ -871,15 +882,15 @@ static void best_edge(uint32_t s1, uint32_t s2, const hull* h1, const hull* h2, 
 
 /* collision_distance_between_skew_lines (clipping.cpp:210-239) */
 static int skew_lines(v3 p1, v3 d1, v3 p2, v3 d2, v3* l1, v3* l2) {
-    double n1 = d1.x * d2.x + d1.y * d2.y + d1.z * d2.z;
-    double n2 = d2.x * d2.x + d2.y * d2.y + d2.z * d2.z;
-    double m1 = -d1.x * d1.x - d1.y * d1.y - d1.z * d1.z;
-    double m2 = -d2.x * d1.x - d2.y * d1.y - d2.z * d1.z;
-    double r1 = -d1.x * p2.x + d1.x * p1.x - d1.y * p2.y + d1.y * p1.y - d1.z * p2.z + d1.z * p1.z;
-    double r2 = -d2.x * p2.x + d2.x * p1.x - d2.y * p2.y + d2.y * p1.y - d2.z * p2.z + d2.z * p1.z;
+    real n1 = d1.x * d2.x + d1.y * d2.y + d1.z * d2.z;
+    real n2 = d2.x * d2.x + d2.y * d2.y + d2.z * d2.z;
+    real m1 = -d1.x * d1.x - d1.y * d1.y - d1.z * d1.z;
+    real m2 = -d2.x * d1.x - d2.y * d1.y - d2.z * d1.z;
+    real r1 = -d1.x * p2.x + d1.x * p1.x - d1.y * p2.y + d1.y * p1.y - d1.z * p2.z + d1.z * p1.z;
+    real r2 = -d2.x * p2.x + d2.x * p1.x - d2.y * p2.y + d2.y * p1.y - d2.z * p2.z + d2.z * p1.z;
     if ((n1 * m2) - (n2 * m1) == 0) return 0;
-    double n = ((r1 * m2) - (r2 * m1)) / ((n1 * m2) - (n2 * m1));
-    double m = ((n1 * r2) - (n2 * r1)) / ((n1 * m2) - (n2 * m1));
+    real n = ((r1 * m2) - (r2 * m1)) / ((n1 * m2) - (n2 * m1));
+    real m = ((n1 * r2) - (n2 * r1)) / ((n1 * m2) - (n2 * m1));
     *l1 = add(p1, scl(m, d1));
     *l2 = add(p2, scl(n, d2));
     return 1;
@@ -899,9 +910,9 @@ static void manifold(const hull* h1, const hull* h2, v3 normal, contactarr* out,
     uint32_t f2 = best_face(s2, h2, inormal, err);
     uint32_t edges[4] = {0, 0, 0, 0};
     best_edge(s1, s2, h1, h2, normal, &edge_n, edges, err);
-    double d1 = dot(h1->tn[f1], normal);
-    double d2 = dot(h2->tn[f2], inormal);
-    double de = dot(edge_n, normal);
+    real d1 = dot(h1->tn[f1], normal);
+    real d2 = dot(h2->tn[f2], inormal);
+    real de = dot(edge_n, normal);
     if (de > d1 + EPSILON && de > d2 + EPSILON) {
         v3 l1, l2;
         v3 p1 = h1->tv[edges[0]];
@@ -937,7 +948,7 @@ static void manifold(const hull* h1, const hull* h2, v3 normal, contactarr* out,
             v3 pt = final_pts.d[i];
             v3 cp = closest_on_plane(pt, &rp);
             v3 diff = sub(pt, cp);
-            double pen;
+            real pen;
             contact c;
             if (ref1) {
                 pen = dot(diff, normal);
@@ -960,7 +971,7 @@ static void manifold(const hull* h1, const hull* h2, v3 normal, contactarr* out,
 static void pair_contacts(const hull* h1, const hull* h2, contactarr* out, uint32_t* err, uint64_t* epa_fail) {
     simplex s;
     if (gjk(h1, h2, &s, err)) {
-        v3 n; double pen;
+        v3 n; real pen;
         if (!epa(h1, h2, &s, &n, &pen, err)) { ++*epa_fail; return; }
         manifold(h1, h2, n, out, err);
     }
@@ -996,19 +1007,19 @@ static void pos_prep(body* e1, body* e2, v3 r1_lc, v3 r2_lc, pcpd* p) {
     p->I2 = dyn_inv_inertia(e2);
 }
 
-static double pos_dlambda(const pcpd* p, double h, double compliance, double lambda, v3 dx, uint32_t* err) {
-    double c = len(dx);
+static real pos_dlambda(const pcpd* p, real h, real compliance, real lambda, v3 dx, uint32_t* err) {
+    real c = len(dx);
     if (c <= 1e-50) return 0.0;
     v3 n = V(dx.x / c, dx.y / c, dx.z / c);
-    double w1 = p->e1->inv_mass + dot(cross(p->r1, n), m3_mulv(&p->I1, cross(p->r1, n)));
-    double w2 = p->e2->inv_mass + dot(cross(p->r2, n), m3_mulv(&p->I2, cross(p->r2, n)));
+    real w1 = p->e1->inv_mass + dot(cross(p->r1, n), m3_mulv(&p->I1, cross(p->r1, n)));
+    real w2 = p->e2->inv_mass + dot(cross(p->r2, n), m3_mulv(&p->I2, cross(p->r2, n)));
     if (w1 + w2 == 0.0) *err |= ORC_ERR_ZERO_MASS;
-    double tc = compliance / (h * h);
+    real tc = compliance / (h * h);
     return (-c - tc * lambda) / (w1 + w2 + tc);
 }
 
-static void pos_apply(const pcpd* p, double dl, v3 dx) {
-    double c = len(dx);
+static void pos_apply(const pcpd* p, real dl, v3 dx) {
+    real c = len(dx);
     if (c <= 1e-50) return;
     body *e1 = p->e1, *e2 = p->e2;
     v3 n = V(dx.x / c, dx.y / c, dx.z / c);
@@ -1038,19 +1049,19 @@ static void ang_prep(body* e1, body* e2, acpd* p) {
     p->I2 = dyn_inv_inertia(e2);
 }
 
-static double ang_dlambda(const acpd* p, double h, double compliance, double lambda, v3 dq, uint32_t* err) {
-    double theta = len(dq);
+static real ang_dlambda(const acpd* p, real h, real compliance, real lambda, v3 dq, uint32_t* err) {
+    real theta = len(dq);
     if (theta <= 1e-50) return 0.0;
     v3 n = V(dq.x / theta, dq.y / theta, dq.z / theta);
-    double w1 = dot(n, m3_mulv(&p->I1, n));
-    double w2 = dot(n, m3_mulv(&p->I2, n));
+    real w1 = dot(n, m3_mulv(&p->I1, n));
+    real w2 = dot(n, m3_mulv(&p->I2, n));
     if (w1 + w2 == 0.0) *err |= ORC_ERR_ZERO_MASS;
-    double tc = compliance / (h * h);
+    real tc = compliance / (h * h);
     return (-theta - tc * lambda) / (w1 + w2 + tc);
 }
 
-static void ang_apply(const acpd* p, double dl, v3 dq) {
-    double theta = len(dq);
+static void ang_apply(const acpd* p, real dl, v3 dq) {
+    real theta = len(dq);
     if (theta <= 1e-50) return;
     body *e1 = p->e1, *e2 = p->e2;
     v3 n = V(dq.x / theta, dq.y / theta, dq.z / theta);
@@ -1076,34 +1087,34 @@ static void ang_apply(const acpd* p, double dl, v3 dq) {
 /* Constraint solves (pbd.cpp:81-406)                                  */
 /* ------------------------------------------------------------------ */
 /* positional_constraint_solve (pbd.cpp:81-97) */
-static void solve_positional(constraint* c, body* e1, body* e2, double h, uint32_t* err) {
+static void solve_positional(constraint* c, body* e1, body* e2, real h, uint32_t* err) {
     v3 dx = sub(sub(e1->x, e2->x), c->distance);
     pcpd p;
     pos_prep(e1, e2, c->r1, c->r2, &p);
-    double dl = pos_dlambda(&p, h, c->compliance, c->lambda, dx, err);
+    real dl = pos_dlambda(&p, h, c->compliance, c->lambda, dx, err);
     pos_apply(&p, dl, dx);
     c->lambda += dl;
 }
 
 /* collision_constraint_solve (pbd.cpp:107-154) */
-static void solve_collision(constraint* c, body* e1, body* e2, double h, uint32_t* err) {
+static void solve_collision(constraint* c, body* e1, body* e2, real h, uint32_t* err) {
     pcpd p;
     pos_prep(e1, e2, c->r1, c->r2, &p);
     v3 p1 = add(e1->x, p.r1);
     v3 p2 = add(e2->x, p.r2);
-    double d = dot(sub(p1, p2), c->n);
+    real d = dot(sub(p1, p2), c->n);
     if (d > 0.0) {
         v3 dx = scl(d, c->n);
-        double dl = pos_dlambda(&p, h, 0.0, c->lambda_n, dx, err);
+        real dl = pos_dlambda(&p, h, 0.0, c->lambda_n, dx, err);
         pos_apply(&p, dl, dx);
         c->lambda_n += dl;
         pos_prep(e1, e2, c->r1, c->r2, &p);
         p1 = add(e1->x, p.r1);
         p2 = add(e2->x, p.r2);
         dl = pos_dlambda(&p, h, 0.0, c->lambda_t, dx, err);
-        const double mu_s = (e1->mu_s + e2->mu_s) / 2.0f;
-        double lambda_n = c->lambda_n;
-        double lambda_t = c->lambda_t + dl;
+        const real mu_s = (e1->mu_s + e2->mu_s) / 2.0f;
+        real lambda_n = c->lambda_n;
+        real lambda_t = c->lambda_t + dl;
         if (lambda_t > mu_s * lambda_n) {
             v3 p1_til = add(e1->px, q_rot(e1->pq, c->r1));
             v3 p2_til = add(e2->px, q_rot(e2->pq, c->r2));
@@ -1116,20 +1127,20 @@ static void solve_collision(constraint* c, body* e1, body* e2, double h, uint32_
 }
 
 /* mutual_orientation_constraint_solve (pbd.cpp:156-173) */
-static void solve_mutual(constraint* c, body* e1, body* e2, double h, uint32_t* err) {
+static void solve_mutual(constraint* c, body* e1, body* e2, real h, uint32_t* err) {
     acpd a;
     ang_prep(e1, e2, &a);
     qt q2i = q_inv(e2->q);
     qt aux = q_mul(e1->q, q2i);
     v3 dq = V(2.0 * aux.x, 2.0 * aux.y, 2.0 * aux.z);
-    double dl = ang_dlambda(&a, h, c->compliance, c->lambda, dq, err);
+    real dl = ang_dlambda(&a, h, c->compliance, c->lambda, dq, err);
     ang_apply(&a, dl, dq);
     c->lambda += dl;
 }
 
 /* limit_angle (pbd.cpp:175-217) */
-static int limit_angle(v3 n, v3 n1, v3 n2, double alpha, double beta, v3* dq) {
-    double phi = asin(dot(cross(n1, n2), n));
+static int limit_angle(v3 n, v3 n1, v3 n2, real alpha, real beta, v3* dq) {
+    real phi = asin(dot(cross(n1, n2), n));
     if (dot(n1, n2) < 0.0) phi = PI_F - phi;
     if (phi > PI_F) phi = phi - 2.0 * PI_F;
     if (phi < -PI_F) phi = phi + 2.0 * PI_F;
@@ -1145,14 +1156,14 @@ static int limit_angle(v3 n, v3 n1, v3 n2, double alpha, double beta, v3* dq) {
 
 /* hinge_joint_constraint_solve (pbd.cpp:245-299); lambda = aligned axes,
  * lambda2 = position, lambda3 = limit */
-static void solve_hinge(constraint* c, body* e1, body* e2, double h, uint32_t* err) {
+static void solve_hinge(constraint* c, body* e1, body* e2, real h, uint32_t* err) {
     int bad = 0;
     acpd a;
     ang_prep(e1, e2, &a);
     v3 a1 = q_axis(e1->q, c->axis[0], &bad);
     v3 a2 = q_axis(e2->q, c->axis[1], &bad);
     v3 dq = cross(a1, a2);
-    double dl = ang_dlambda(&a, h, c->compliance, c->lambda, dq, err);
+    real dl = ang_dlambda(&a, h, c->compliance, c->lambda, dq, err);
     ang_apply(&a, dl, dq);
     c->lambda += dl;
     pcpd p;
@@ -1170,7 +1181,7 @@ static void solve_hinge(constraint* c, body* e1, body* e2, double h, uint32_t* e
         if (limit_angle(n, n1, n2, c->limits[0], c->limits[1], &dq)) {
             acpd a2p;
             ang_prep(e1, e2, &a2p);
-            double dl2 = ang_dlambda(&a2p, h, 0.0, c->lambda3, dq, err);
+            real dl2 = ang_dlambda(&a2p, h, 0.0, c->lambda3, dq, err);
             ang_apply(&a2p, dl2, dq);
             c->lambda3 += dl2;
         }
@@ -1180,7 +1191,7 @@ static void solve_hinge(constraint* c, body* e1, body* e2, double h, uint32_t* e
 
 /* spherical_joint_constraint_solve (pbd.cpp:301-379); lambda = position,
  * lambda2 = swing, lambda3 = twist */
-static void solve_spherical(constraint* c, body* e1, body* e2, double h, uint32_t* err) {
+static void solve_spherical(constraint* c, body* e1, body* e2, real h, uint32_t* err) {
     const double EPSILON = 1e-50;
     int bad = 0;
     pcpd p;
@@ -1188,21 +1199,21 @@ static void solve_spherical(constraint* c, body* e1, body* e2, double h, uint32_
     v3 p1 = add(e1->x, p.r1);
     v3 p2 = add(e2->x, p.r2);
     v3 dx = sub(p1, p2);
-    double dl = pos_dlambda(&p, h, 0.0, c->lambda, dx, err);
+    real dl = pos_dlambda(&p, h, 0.0, c->lambda, dx, err);
     pos_apply(&p, dl, dx);
     c->lambda += dl;
 
     v3 n1 = q_axis(e1->q, c->axis[0], &bad);
     v3 n2 = q_axis(e2->q, c->axis[1], &bad);
     v3 n = cross(n1, n2);
-    double n_len = len(n);
+    real n_len = len(n);
     if (n_len > EPSILON) {
         n = V(n.x / n_len, n.y / n_len, n.z / n_len);
         v3 dq;
         if (limit_angle(n, n1, n2, c->limits[0], c->limits[1], &dq)) {
             acpd a;
             ang_prep(e1, e2, &a);
-            double d2 = ang_dlambda(&a, h, 0.0, c->lambda2, dq, err);
+            real d2 = ang_dlambda(&a, h, 0.0, c->lambda2, dq, err);
             ang_apply(&a, d2, dq);
             c->lambda2 += d2;
         }
@@ -1217,8 +1228,8 @@ static void solve_spherical(constraint* c, body* e1, body* e2, double h, uint32_
         n = V(n.x / n_len, n.y / n_len, n.z / n_len);
         n1 = sub(b1, scl(dot(n, b1), n));
         n2 = sub(b2, scl(dot(n, b2), n));
-        double n1_len = len(n1);
-        double n2_len = len(n2);
+        real n1_len = len(n1);
+        real n2_len = len(n2);
         if (n1_len > EPSILON && n2_len > EPSILON) {
             n1 = V(n1.x / n1_len, n1.y / n1_len, n1.z / n1_len);
             n2 = V(n2.x / n2_len, n2.y / n2_len, n2.z / n2_len);
@@ -1226,7 +1237,7 @@ static void solve_spherical(constraint* c, body* e1, body* e2, double h, uint32_
             if (limit_angle(n, n1, n2, c->limits[2], c->limits[3], &dq)) {
                 acpd a;
                 ang_prep(e1, e2, &a);
-                double d3 = ang_dlambda(&a, h, 0.0, c->lambda3, dq, err);
+                real d3 = ang_dlambda(&a, h, 0.0, c->lambda3, dq, err);
                 ang_apply(&a, d3, dq);
                 c->lambda3 += d3;
             }
@@ -1235,7 +1246,7 @@ static void solve_spherical(constraint* c, body* e1, body* e2, double h, uint32_
     if (bad) *err |= ORC_ERR_SELECT;
 }
 
-static void solve_constraint(scene* sc, constraint* c, double h) {
+static void solve_constraint(scene* sc, constraint* c, real h) {
     body* e1 = &sc->b[c->b1];
     body* e2 = &sc->b[c->b2];
     switch (c->type) {
@@ -1249,28 +1260,28 @@ static void solve_constraint(scene* sc, constraint* c, double h) {
 }
 
 /* Velocity solve of one collision constraint (pbd.cpp:646-711) */
-static void velocity_solve(constraint* c, body* e1, body* e2, double h) {
+static void velocity_solve(constraint* c, body* e1, body* e2, real h) {
     v3 n = c->n;
-    double lambda_n = c->lambda_n;
+    real lambda_n = c->lambda_n;
     pcpd p;
     pos_prep(e1, e2, c->r1, c->r2, &p);
     v3 v1 = e1->v, w1 = e1->w, v2 = e2->v, w2 = e2->w;
     v3 v = sub(add(v1, cross(w1, p.r1)), add(v2, cross(w2, p.r2)));
-    double vn = dot(n, v);
+    real vn = dot(n, v);
     v3 vt = sub(v, scl(vn, n));
     v3 dv = V(0.0, 0.0, 0.0);
-    const double mu_d = (e1->mu_d + e2->mu_d) / 2.0f;
-    double fn = lambda_n / h;
-    double fact = RP_MIN(mu_d * fabs(fn), len(vt));
+    const real mu_d = (e1->mu_d + e2->mu_d) / 2.0f;
+    real fn = lambda_n / h;
+    real fact = RP_MIN(mu_d * fabs(fn), len(vt));
     dv = add(dv, scl(-fact, nrm(vt)));
     v3 ov1 = e1->pv, ow1 = e1->pw, ov2 = e2->pv, ow2 = e2->pw;
     v3 vtil = sub(add(ov1, cross(ow1, p.r1)), add(ov2, cross(ow2, p.r2)));
-    double vn_til = dot(n, vtil);
-    double e = e1->e * e2->e;
+    real vn_til = dot(n, vtil);
+    real e = e1->e * e2->e;
     fact = -vn + RP_MIN(-e * vn_til, 0.0);
     dv = add(dv, scl(fact, n));
-    double _w1 = e1->inv_mass + dot(cross(p.r1, n), m3_mulv(&p.I1, cross(p.r1, n)));
-    double _w2 = e2->inv_mass + dot(cross(p.r2, n), m3_mulv(&p.I2, cross(p.r2, n)));
+    real _w1 = e1->inv_mass + dot(cross(p.r1, n), m3_mulv(&p.I1, cross(p.r1, n)));
+    real _w2 = e2->inv_mass + dot(cross(p.r2, n), m3_mulv(&p.I2, cross(p.r2, n)));
     v3 pimp = scl(1.0 / (_w1 + _w2), dv);
     if (!e1->fixed) {
         e1->v = add(e1->v, scl(e1->inv_mass, pimp));
@@ -1311,8 +1322,8 @@ static uint32_t broad(const scene* sc, uint32_t** out) {
         const body* e1 = &sc->b[i];
         for (uint32_t j = i + 1; j < sc->nb; ++j) {
             const body* e2 = &sc->b[j];
-            double dd = len(sub(e1->x, e2->x));
-            double maxd = e1->radius + e2->radius + 0.1;
+            real dd = len(sub(e1->x, e2->x));
+            real maxd = e1->radius + e2->radius + 0.1;
             if (dd <= maxd) { ARR_PUSH(pairs, i); ARR_PUSH(pairs, j); }
         }
     }
@@ -1334,7 +1345,7 @@ static void uf_union(uint32_t* parent, uint32_t x, uint32_t y) {
 /* Islands + sleeping (broad.cpp:31-116, pbd.cpp:476-506). The reference
  * groups bodies by union-find root; the active flags depend only on the
  * partition, so the union-find here keeps indices instead of a hash map. */
-static void islands_and_sleep(scene* sc, const uint32_t* pairs, uint32_t np, double dt) {
+static void islands_and_sleep(scene* sc, const uint32_t* pairs, uint32_t np, real dt) {
     uint32_t* parent = malloc(sizeof(uint32_t) * (sc->nb ? sc->nb : 1));
     for (uint32_t i = 0; i < sc->nb; ++i) parent[i] = i;
     for (uint32_t k = 0; k < np; ++k) {
@@ -1350,8 +1361,8 @@ static void islands_and_sleep(scene* sc, const uint32_t* pairs, uint32_t np, dou
     for (uint32_t i = 0; i < sc->nb; ++i) {
         body* e = &sc->b[i];
         if (e->fixed) continue;
-        double lv = len(e->v);
-        double av = len(e->w);
+        real lv = len(e->v);
+        real av = len(e->w);
         if (lv < 0.10 && av < 0.10) e->deact += dt;
         else e->deact = 0.0;
         if (e->deact < 1.0) all_inactive[uf_find(parent, i)] = 0;
@@ -1366,9 +1377,10 @@ static void islands_and_sleep(scene* sc, const uint32_t* pairs, uint32_t np, dou
 }
 
 /* pbd_simulate_with_constraints (pbd.cpp:468-747) for one scene */
-static void simulate(orc_world* w, scene* sc, double dt, uint32_t substeps, uint32_t iters, int32_t collisions) {
+static void simulate(orc_world* w, scene* sc, real dt, uint32_t substeps, uint32_t iters, int32_t collisions) {
     if (dt <= 0.0) return;
-    double h = dt / substeps;
+    ORC_PHASE(PH_FRAME);
+    real h = dt / substeps;
     uint32_t* pairs = NULL;
     uint32_t np = broad(sc, &pairs);
     islands_and_sleep(sc, pairs, np, dt);
@@ -1376,6 +1388,7 @@ static void simulate(orc_world* w, scene* sc, double dt, uint32_t substeps, uint
     contactarr cts = {0};
     for (uint32_t s = 0; s < substeps; ++s) {
         /* integrate (pbd.cpp:537-577) */
+        ORC_PHASE(PH_INTEGRATE);
         for (uint32_t j = 0; j < sc->nb; ++j) {
             body* e = &sc->b[j];
             e->px = e->x;
@@ -1409,7 +1422,9 @@ static void simulate(orc_world* w, scene* sc, double dt, uint32_t substeps, uint
              * transforms both bodies of each pair right before use
              * (pbd.cpp:598-599); poses do not change between pairs, so the
              * values are identical. */
+            ORC_PHASE(PH_TRANSFORM);
             for (uint32_t j = 0; j < sc->nb; ++j) hull_transform(&sc->b[j].h, sc->b[j].x, sc->b[j].q);
+            ORC_PHASE(PH_NARROW);
             for (uint32_t k = 0; k < np; ++k) {
                 uint32_t i1 = pairs[2 * k], i2 = pairs[2 * k + 1];
                 body *e1 = &sc->b[i1], *e2 = &sc->b[i2];
@@ -1427,9 +1442,11 @@ static void simulate(orc_world* w, scene* sc, double dt, uint32_t substeps, uint
             }
         }
         /* position solve (pbd.cpp:615-620) */
+        ORC_PHASE(PH_POSITION);
         for (uint32_t it = 0; it < iters; ++it)
             for (uint32_t k = 0; k < cons.n; ++k) solve_constraint(sc, &cons.d[k], h);
         /* velocity update (pbd.cpp:623-643) */
+        ORC_PHASE(PH_VELOCITY_UPDATE);
         for (uint32_t j = 0; j < sc->nb; ++j) {
             body* e = &sc->b[j];
             if (e->fixed) continue;
@@ -1443,6 +1460,7 @@ static void simulate(orc_world* w, scene* sc, double dt, uint32_t substeps, uint
             else e->w = scl(-2.0 / h, V(dq.x, dq.y, dq.z));
         }
         /* velocity solve (pbd.cpp:646-711) */
+        ORC_PHASE(PH_VELOCITY_SOLVE);
         for (uint32_t k = 0; k < cons.n; ++k) {
             constraint* c = &cons.d[k];
             if (c->type == ORC_COLLISION_CONSTRAINT) velocity_solve(c, &sc->b[c->b1], &sc->b[c->b2], h);
@@ -1461,10 +1479,11 @@ static qt ld4(const double* p, uint32_t i) {
     qt q; q.x = p[4 * i]; q.y = p[4 * i + 1]; q.z = p[4 * i + 2]; q.w = p[4 * i + 3];
     return q;
 }
-static void st3(double* p, uint32_t i, v3 v) { p[3 * i] = v.x; p[3 * i + 1] = v.y; p[3 * i + 2] = v.z; }
-static void st4(double* p, uint32_t i, qt q) { p[4 * i] = q.x; p[4 * i + 1] = q.y; p[4 * i + 2] = q.z; p[4 * i + 3] = q.w; }
+static void st3(double* p, uint32_t i, v3 v) { p[3 * i] = RD(v.x); p[3 * i + 1] = RD(v.y); p[3 * i + 2] = RD(v.z); }
+static void st4(double* p, uint32_t i, qt q) { p[4 * i] = RD(q.x); p[4 * i + 1] = RD(q.y); p[4 * i + 2] = RD(q.z); p[4 * i + 3] = RD(q.w); }
 
 orc_world* orc_world_create(const orc_world_desc* d) {
+    ORC_PHASE(PH_SETUP);
     orc_world* w = calloc(1, sizeof(orc_world));
     w->ns = d->num_scenes;
     w->s = calloc(w->ns ? w->ns : 1, sizeof(scene));
@@ -1500,9 +1519,9 @@ orc_world* orc_world_create(const orc_world_desc* d) {
             e->px = V(0.0, 0.0, 0.0);
             e->pq.x = e->pq.y = e->pq.z = e->pq.w = 0.0;
             /* colliders_get_bounding_sphere_radius over one hull (collider.cpp:178-189, 510-521) */
-            double maxd = 0.0;
+            real maxd = 0.0;
             for (uint32_t i = 0; i < e->h.nv; ++i) {
-                double l = len(e->h.v[i]);
+                real l = len(e->h.v[i]);
                 if (l > maxd) maxd = l;
             }
             e->radius = maxd > -DBL_MAX ? maxd : -DBL_MAX;
@@ -1512,10 +1531,10 @@ orc_world* orc_world_create(const orc_world_desc* d) {
                 memset(&e->inertia, 0, sizeof(m3));
                 memset(&e->inv_inertia, 0, sizeof(m3));
             } else {
-                double mass = d->body_mass[g];
+                real mass = d->body_mass[g];
                 e->inv_mass = 1.0 / mass;
                 /* colliders_get_default_inertia_tensor (collider.cpp:448-494) */
-                double mpv = mass / e->h.nv;
+                real mpv = mass / e->h.nv;
                 m3 I;
                 memset(&I, 0, sizeof(I));
                 for (uint32_t i = 0; i < e->h.nv; ++i) {
@@ -1581,6 +1600,7 @@ void orc_world_step(orc_world* w, double dt, uint32_t frames, uint32_t substeps,
     for (uint32_t f = 0; f < frames; ++f) {
         for (uint32_t si = 0; si < w->ns; ++si) {
             scene* sc = &w->s[si];
+            ORC_PHASE(PH_FRAME);
             /* Caller pattern (stack.cpp:93-96) and the force/torque sums
              * (physics_util.cpp:5-23) over the single gravity force. */
             for (uint32_t k = 0; k < sc->nb; ++k) {
@@ -1610,7 +1630,7 @@ void orc_world_get_state(const orc_world* w, const orc_state* o) {
             if (o->previous_linear_velocity) st3(o->previous_linear_velocity, g, e->pv);
             if (o->previous_angular_velocity) st3(o->previous_angular_velocity, g, e->pw);
             if (o->active) o->active[g] = e->active;
-            if (o->deactivation_time) o->deactivation_time[g] = e->deact;
+            if (o->deactivation_time) o->deactivation_time[g] = RD(e->deact);
         }
     }
 }
@@ -1647,10 +1667,10 @@ void orc_world_body_info(const orc_world* w, double* inv_mass, double* inertia, 
     for (uint32_t si = 0; si < w->ns; ++si) {
         for (uint32_t k = 0; k < w->s[si].nb; ++k, ++g) {
             const body* e = &w->s[si].b[k];
-            if (inv_mass) inv_mass[g] = e->inv_mass;
+            if (inv_mass) inv_mass[g] = RD(e->inv_mass);
             if (inertia) memcpy(inertia + 9 * g, e->inertia.m, sizeof(double) * 9);
             if (inv_inertia) memcpy(inv_inertia + 9 * g, e->inv_inertia.m, sizeof(double) * 9);
-            if (radius) radius[g] = e->radius;
+            if (radius) radius[g] = RD(e->radius);
         }
     }
 }
@@ -1677,9 +1697,9 @@ uint32_t orc_world_contacts(orc_world* w, uint32_t si, uint32_t i, uint32_t j, d
     for (uint32_t k = 0; k < cts.n && k < cap; ++k) {
         const contact* c = &cts.d[k];
         double* o = out + 9 * k;
-        o[0] = c->p1.x; o[1] = c->p1.y; o[2] = c->p1.z;
-        o[3] = c->p2.x; o[4] = c->p2.y; o[5] = c->p2.z;
-        o[6] = c->n.x; o[7] = c->n.y; o[8] = c->n.z;
+        o[0] = RD(c->p1.x); o[1] = RD(c->p1.y); o[2] = RD(c->p1.z);
+        o[3] = RD(c->p2.x); o[4] = RD(c->p2.y); o[5] = RD(c->p2.z);
+        o[6] = RD(c->n.x); o[7] = RD(c->n.y); o[8] = RD(c->n.z);
     }
     uint32_t n = cts.n;
     ARR_FREE(cts);

@@ -51,6 +51,8 @@ def library():
         "rp_pbd_simulate": (u32, [f64, _P, _P, u32, u32, i32]),
         "physics_simulate": (None, [_P, f64]),
         "rp_fp64_selftest": (u32, [_P, _P, _P, u64]),
+        "rp_world_set_profiling": (None, [_P, i32]),
+        "rp_world_kernel_times": (None, [_P, _P, _P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -114,6 +116,18 @@ class World:
 
     def epa_failures(self) -> int:
         return int(self.lib.rp_world_epa_failures(self.h))
+
+    KERNEL_CLASSES = ("frame", "integrate", "transform", "narrow", "solve")
+
+    def set_profiling(self, on: bool):
+        self.lib.rp_world_set_profiling(self.h, 1 if on else 0)
+
+    def kernel_times(self) -> dict:
+        """{class: (total ms, launches)} recorded in profiling mode."""
+        ms = np.zeros(8)
+        n = np.zeros(8, dtype=np.uint64)
+        self.lib.rp_world_kernel_times(self.h, ms.ctypes.data, n.ctypes.data)
+        return {c: (float(ms[i]), int(n[i])) for i, c in enumerate(self.KERNEL_CLASSES)}
 
     @property
     def stream(self) -> int:

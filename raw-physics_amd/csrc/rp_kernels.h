@@ -1,6 +1,7 @@
 // rp_kernels.h — device-resident world layout and kernel launchers.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <vector>
 #include "rp_step.h"
 
 namespace rp {
@@ -59,7 +60,37 @@ struct FrameParams {
     bool forces_given = false;  // body F/T already hold the frame's force sums
 };
 
-hipError_t launch_frame(const DevWorld& w, const FrameParams& fp, hipStream_t st);
+// Kernel classes timed by the profiling mode.
+enum { KC_FRAME = 0, KC_INTEGRATE, KC_TRANSFORM, KC_NARROW, KC_SOLVE, KC_COUNT = 8 };
+
+// Records a pair of HIP events around each timed launch (profiling mode
+// only); the host reads the elapsed times after the frame completes.
+struct KernelTimer {
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    std::vector<int> cls;  // kernel class of each recorded pair
+    size_t used = 0;
+    void begin(int c, hipStream_t st) {
+        if (!on) return;
+        if (pool.size() < 2 * (used + 1)) {
+            hipEvent_t a, b;
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            pool.push_back(a);
+            pool.push_back(b);
+        }
+        if (cls.size() <= used) cls.resize(used + 1);
+        cls[used] = c;
+        (void)hipEventRecord(pool[2 * used], st);
+    }
+    void end(hipStream_t st) {
+        if (!on) return;
+        (void)hipEventRecord(pool[2 * used + 1], st);
+        ++used;
+    }
+};
+
+hipError_t launch_frame(const DevWorld& w, const FrameParams& fp, hipStream_t st, KernelTimer* kt = nullptr);
 hipError_t launch_pair_contacts(const DevWorld& w, int b1, int b2, double* out, int* count, uint32_t* err,
                                 hipStream_t st);
 hipError_t launch_broad(const DevWorld& w, hipStream_t st);

@@ -434,20 +434,32 @@ __global__ void k_fp64_selftest(const double* a, const double* b, double* out, i
 // ---------------------------------------------------------------------------
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
-hipError_t launch_frame(const DevWorld& w, const FrameParams& fp, hipStream_t st) {
+hipError_t launch_frame(const DevWorld& w, const FrameParams& fp, hipStream_t st, KernelTimer* kt) {
     if (w.nb == 0) return hipSuccess;
+    KernelTimer none;
+    KernelTimer& t = kt ? *kt : none;
+    t.begin(KC_FRAME, st);
     if (!fp.forces_given) hipLaunchKernelGGL(k_frame_begin, dim3(cdiv(w.nb, 256)), dim3(256), 0, st, w, fp.gravity);
     hipLaunchKernelGGL(k_broad, dim3(w.ns), dim3(64), 0, st, w);
     hipLaunchKernelGGL(k_islands, dim3(w.ns), dim3(64), 0, st, w, fp.dt);
     hipLaunchKernelGGL(k_scan_live, dim3(1), dim3(1024), 0, st, w);
     hipLaunchKernelGGL(k_scatter_live, dim3(w.ns), dim3(64), 0, st, w);
+    t.end(st);
     for (unsigned sub = 0; sub < fp.substeps; ++sub) {
+        t.begin(KC_INTEGRATE, st);
         hipLaunchKernelGGL(k_integrate, dim3(cdiv(w.nb, 128)), dim3(128), 0, st, w, fp.h);
+        t.end(st);
         if (fp.collisions) {
+            t.begin(KC_TRANSFORM, st);
             hipLaunchKernelGGL(k_transform, dim3(cdiv(w.n_xf, 256)), dim3(256), 0, st, w);
+            t.end(st);
+            t.begin(KC_NARROW, st);
             hipLaunchKernelGGL(k_narrow, dim3(w.narrow_blocks), dim3(64), 0, st, w);
+            t.end(st);
         }
+        t.begin(KC_SOLVE, st);
         hipLaunchKernelGGL(k_solve, dim3(w.ns), dim3(64), 0, st, w, fp.h, (int)fp.iters, fp.collisions ? 1 : 0);
+        t.end(st);
     }
     return hipGetLastError();
 }

@@ -501,6 +501,9 @@ struct rp_world {
     DeviceImage im;
     hipStream_t stream = nullptr;
     std::vector<Body> host_bodies;  // scratch for state I/O
+    KernelTimer timer;              // profiling mode
+    double kc_ms[KC_COUNT] = {0};
+    uint64_t kc_launches[KC_COUNT] = {0};
 };
 
 static bool world_init(rp_world* w) {
@@ -514,6 +517,7 @@ static bool world_init(rp_world* w) {
 static void world_free(rp_world* w) {
     if (!w) return;
     if (w->stream) (void)hipStreamSynchronize(w->stream);
+    for (hipEvent_t e : w->timer.pool) (void)hipEventDestroy(e);
     w->im.release();
     if (w->stream) (void)hipStreamDestroy(w->stream);
     delete w;
@@ -521,10 +525,21 @@ static void world_free(rp_world* w) {
 
 static uint32_t world_run(rp_world* w, const FrameParams& fp, uint32_t frames, bool sync) {
     for (uint32_t f = 0; f < frames; ++f) {
-        hipError_t e = launch_frame(w->im.d, fp, w->stream);
+        w->timer.used = 0;
+        hipError_t e = launch_frame(w->im.d, fp, w->stream, &w->timer);
         if (e != hipSuccess) {
             set_msg(std::string("kernel launch: ") + hipGetErrorString(e));
             return RP_ERR_RUNTIME;
+        }
+        if (w->timer.on) {  // profiling: wait for the frame, then read its kernel times
+            if (hipStreamSynchronize(w->stream) != hipSuccess) return RP_ERR_RUNTIME;
+            for (size_t k = 0; k < w->timer.used; ++k) {
+                float ms = 0.0f;
+                if (hipEventElapsedTime(&ms, w->timer.pool[2 * k], w->timer.pool[2 * k + 1]) == hipSuccess) {
+                    w->kc_ms[w->timer.cls[k]] += ms;
+                    ++w->kc_launches[w->timer.cls[k]];
+                }
+            }
         }
     }
     if (sync) {
@@ -736,6 +751,19 @@ uint32_t rp_fp64_selftest(const double* a, const double* b, double* out, uint64_
     if (db) (void)hipFree(db);
     if (dout) (void)hipFree(dout);
     return rc;
+}
+
+void rp_world_set_profiling(rp_world* w, int32_t on) {
+    (void)hipStreamSynchronize(w->stream);
+    w->timer.on = on != 0;
+    for (int k = 0; k < KC_COUNT; ++k) { w->kc_ms[k] = 0.0; w->kc_launches[k] = 0; }
+}
+
+void rp_world_kernel_times(rp_world* w, double* ms, uint64_t* launches) {
+    for (int k = 0; k < KC_COUNT; ++k) {
+        if (ms) ms[k] = w->kc_ms[k];
+        if (launches) launches[k] = w->kc_launches[k];
+    }
 }
 
 uint32_t rp_last_error(void) { return g_last_err; }

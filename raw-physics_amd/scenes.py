@@ -56,12 +56,27 @@ def _corners(verts, faces) -> np.ndarray:
     return out
 
 
+_CUBE = None
+
+
 def cube_corners() -> np.ndarray:
     """Corner stream of res/cube.obj (36 corners, 8 unique vertices)."""
-    return _corners(_CUBE_V, _CUBE_F)
+    global _CUBE
+    if _CUBE is None:
+        _CUBE = _corners(_CUBE_V, _CUBE_F)
+    return _CUBE
+
+
+_ICO_CACHE = {}
 
 
 def icosphere_corners(subdivisions: int) -> np.ndarray:
+    if subdivisions not in _ICO_CACHE:
+        _ICO_CACHE[subdivisions] = _icosphere_corners(subdivisions)
+    return _ICO_CACHE[subdivisions]
+
+
+def _icosphere_corners(subdivisions: int) -> np.ndarray:
     """Unit-radius icosphere: res/ico.obj's icosahedron with every triangle
     split at its edge midpoints `subdivisions` times, vertices projected to
     the unit sphere (12 / 42 / 162 vertices for 0 / 1 / 2 subdivisions)."""
@@ -341,12 +356,15 @@ def drop_scene(w: World, seed: int, n_cubes: int = 8, n_icos: int = 8, ico_subdi
                grid: int = 4) -> World:
     """One batched-uniform scene: static floor + `n_cubes` cubes (half-extent r)
     + `n_icos` icospheres (radius r), r ~ U(0.5, 1.0), mass = hull volume
-    (density 1), dropped from a grid x grid lattice at heights U(1, 6) above
-    the floor top, uniformly random orientation, v ~ N(0, 0.5)^3, w = 0."""
+    (density 1), dropped from a grid x grid lattice with their lowest point
+    U(1, 6) above the floor top, uniformly random orientation,
+    v ~ N(0, 0.5)^3, w = 0."""
     rng = Rng(0x5EED0000 + seed)
     _floor(w)
     cube = w.shape(cube_corners())
     ico = w.shape(icosphere_corners(ico_subdiv))
+    unit_volume = {cube: hull_volume(w.shapes[cube], (1.0, 1.0, 1.0)),
+                   ico: hull_volume(w.shapes[ico], (1.0, 1.0, 1.0))}
     n = n_cubes + n_icos
     for k in range(n):
         is_cube = k < n_cubes
@@ -355,12 +373,14 @@ def drop_scene(w: World, seed: int, n_cubes: int = 8, n_icos: int = 8, ico_subdi
         layer = k // (grid * grid)
         x = (gx - (grid - 1) / 2.0) * GRID_SPACING
         z = (gz - (grid - 1) / 2.0) * GRID_SPACING
-        y = -1.0 + rng.uniform(1.0, 6.0) + 7.0 * layer
+        # lowest point of the body 1..6 above the floor top (y = -1)
+        bound = math.sqrt(3.0) * r if is_cube else r
+        y = -1.0 + bound + rng.uniform(1.0, 6.0) + 8.0 * layer
         q = rng.quaternion()
         v = (rng.normal(0.5), rng.normal(0.5), rng.normal(0.5))
         shape = cube if is_cube else ico
         scale = (r, r, r)
-        mass = hull_volume(w.shapes[shape], scale)
+        mass = unit_volume[shape] * r * r * r
         w.add_body(shape, scale, (x, y, z), q, mass=mass, material=(0.5, 0.3, 0.0), linear_velocity=v)
     w.end_scene()
     return w
@@ -370,4 +390,72 @@ def batched_uniform(num_scenes: int, first_seed: int = 0, **kw) -> World:
     w = World(name=f"batched_uniform_{num_scenes}")
     for s in range(num_scenes):
         drop_scene(w, first_seed + s, **kw)
+    return w
+
+
+def _qrot(q, v):
+    """quaternion_apply_to_vec3 (src/quaternion.cpp:257-268), for placing links."""
+    x, y, z, w = q
+    ix = w * v[0] + y * v[2] - z * v[1]
+    iy = w * v[1] + z * v[0] - x * v[2]
+    iz = w * v[2] + x * v[1] - y * v[0]
+    iw = -x * v[0] - y * v[1] - z * v[2]
+    return ((ix * w) + (iw * -x) + (iy * -z) - (iz * -y),
+            (iy * w) + (iw * -y) + (iz * -x) - (ix * -z),
+            (iz * w) + (iw * -z) + (ix * -y) - (iy * -x))
+
+
+def chain_scene(w: World, seed: int, length: int = 6, with_floor: bool = True, link_scale=(0.2, 0.5, 0.1),
+                velocity_sigma: float = 1.0) -> World:
+    """A static support with a chain of `length` box links hanging from it,
+    joined alternately by limited hinge joints (aligned axis +z, limit axis
+    +y, limits +-pi/2) and spherical joints (swing axis +x, twist axis +y,
+    swing +-pi/4, twist +-pi/8), as in the reference's arm scene
+    (src/examples/arm.cpp:44-110). Pivots are the link ends with seeded
+    jitter; each link is moved so that its pivot meets the previous link's
+    (reset_joint_distance, arm.cpp:32-42). Links start with v ~ N(0, sigma)."""
+    rng = Rng(0x5EED0000 + seed)
+    cube = w.shape(cube_corners())
+    if with_floor:
+        _floor(w)
+    ident = quaternion_new((1.0, 0.0, 0.0), 0.0)
+    sup_pos = (0.0, 1.0 + 2.0 * link_scale[1] * length + 0.5, 0.0)
+    prev = w.add_body(cube, (0.2, 0.1, 0.1), sup_pos, ident, fixed=True, material=(0.5, 0.5, 0.0))
+    prev_pos, prev_q, prev_half = sup_pos, ident, 0.1
+    for k in range(length):
+        jit = lambda: rng.uniform(-0.05, 0.05)  # noqa: E731
+        r1 = (jit(), -prev_half + jit(), jit())
+        r2 = (jit(), link_scale[1] + jit(), jit())
+        yaw = rng.uniform(-10.0, 10.0)
+        q = quaternion_new((0.0, 1.0, 0.0), yaw)
+        # place at the pivot: x2 += (x1 + R1 r1) - (x2 + R2 r2) with x2 = x1 initially
+        p1 = tuple(a + b for a, b in zip(prev_pos, _qrot(prev_q, r1)))
+        p2 = tuple(a + b for a, b in zip(prev_pos, _qrot(q, r2)))
+        pos = tuple(x + (a - b) for x, a, b in zip(prev_pos, p1, p2))
+        v = (rng.normal(velocity_sigma), rng.normal(velocity_sigma), rng.normal(velocity_sigma))
+        body = w.add_body(cube, link_scale, pos, q, mass=1.0, material=(0.6, 0.6, 0.0), linear_velocity=v)
+        c = ConstraintDesc()
+        c.body1, c.body2 = prev, body
+        c.r1_lc[:] = r1
+        c.r2_lc[:] = r2
+        if k % 2 == 0:
+            c.type = HINGE
+            c.axis[:] = (POS_Z, POS_Z, POS_Y, POS_Y)
+            c.limited = 1
+            c.compliance = 0.0
+            c.limits[:] = (-PI_F / 2.0, PI_F / 2.0, 0.0, 0.0)
+        else:
+            c.type = SPHERICAL
+            c.axis[:] = (POS_X, POS_X, POS_Y, POS_Y)
+            c.limits[:] = (-PI_F / 4.0, PI_F / 4.0, -PI_F / 8.0, PI_F / 8.0)
+        w.add_constraint(c)
+        prev, prev_pos, prev_q, prev_half = body, pos, q, link_scale[1]
+    w.end_scene()
+    return w
+
+
+def chains(num_scenes: int, length: int = 6, first_seed: int = 0, **kw) -> World:
+    w = World(name=f"chains_{num_scenes}x{length}")
+    for s in range(num_scenes):
+        chain_scene(w, first_seed + s, length=length, **kw)
     return w
