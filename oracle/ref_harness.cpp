@@ -289,3 +289,29 @@ API void ref_world_hull(void* wp, uint32_t g, uint32_t counts[6], double* vertic
     if (vn_off) vn_off[nv] = nvn;
     counts[0] = nv; counts[1] = nf; counts[2] = nfe; counts[3] = nvf; counts[4] = nvn; counts[5] = nff;
 }
+
+/* The same frame driver with another implementation of the step in place of
+ * the reference's pbd_simulate_with_constraints: `fn` has the signature of
+ * rp_pbd_simulate (include/rawphys.h). Used to check the drop-in on entities
+ * and colliders built by the reference's own code. */
+typedef uint32_t (*step_fn)(double, Entity**, Constraint*, uint32_t, uint32_t, int32_t);
+API uint32_t ref_world_step_with(void* wp, void* fn, double dt, uint32_t frames, uint32_t substeps, uint32_t iters,
+                                 int32_t collisions, double gravity) {
+    ref_world* w = (ref_world*)wp;
+    step_fn step = (step_fn)fn;
+    uint32_t err = 0;
+    for (uint32_t f = 0; f < frames; ++f) {
+        for (size_t s = 0; s < w->scenes.size(); ++s) {
+            Entity** ents = w->scenes[s];
+            for (u32 i = 0; i < array_length(ents); ++i)
+                colliders_update(ents[i]->colliders, ents[i]->world_position, &ents[i]->world_rotation);
+            const r64 GRAVITY = gravity;
+            for (u32 i = 0; i < array_length(ents); ++i)
+                entity_add_force(ents[i], (vec3){0.0, 0.0, 0.0}, (vec3){0.0, -GRAVITY * 1.0 / ents[i]->inverse_mass, 0.0},
+                                 false);
+            err |= step(dt, ents, w->constraints[s], substeps, iters, collisions);
+            for (u32 i = 0; i < array_length(ents); ++i) entity_clear_forces(ents[i]);
+        }
+    }
+    return err;
+}

@@ -247,6 +247,7 @@ typedef struct {
     uint32_t *ff_off, *ff; /* face_to_neighbors */
     v3* tv;  /* transformed vertices */
     v3* tn;  /* transformed face normals */
+    int stamp; /* marked for transformation this substep */
 } hull;
 
 typedef struct {
@@ -1418,12 +1419,21 @@ static void simulate(orc_world* w, scene* sc, real dt, uint32_t substeps, uint32
             ARR_PUSH(cons, c);
         }
         if (collisions) {
-            /* Transform every hull once for this substep: the reference
-             * transforms both bodies of each pair right before use
-             * (pbd.cpp:598-599); poses do not change between pairs, so the
-             * values are identical. */
+            /* Transform each hull in a visited pair once for this substep:
+             * the reference transforms both bodies of each pair right before
+             * use (pbd.cpp:598-599); poses do not change between pairs, so
+             * the values are identical. */
             ORC_PHASE(PH_TRANSFORM);
-            for (uint32_t j = 0; j < sc->nb; ++j) hull_transform(&sc->b[j].h, sc->b[j].x, sc->b[j].q);
+            for (uint32_t k = 0; k < np; ++k) {  /* bodies of the pairs visited below, once each */
+                body *e1 = &sc->b[pairs[2 * k]], *e2 = &sc->b[pairs[2 * k + 1]];
+                if ((e1->fixed || !e1->active) && (e2->fixed || !e2->active)) continue;
+                e1->h.stamp = e2->h.stamp = 1;
+            }
+            for (uint32_t j = 0; j < sc->nb; ++j) {
+                if (!sc->b[j].h.stamp) continue;
+                sc->b[j].h.stamp = 0;
+                hull_transform(&sc->b[j].h, sc->b[j].x, sc->b[j].q);
+            }
             ORC_PHASE(PH_NARROW);
             for (uint32_t k = 0; k < np; ++k) {
                 uint32_t i1 = pairs[2 * k], i2 = pairs[2 * k + 1];

@@ -16,7 +16,7 @@ from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
 LIB = HERE / "librawphys.so"
-SOURCES = [HERE / "csrc" / "rp_kernels.hip", HERE / "csrc" / "rp_world.cpp"]
+SOURCES = [HERE / "csrc" / "rp_kernels.hip", HERE / "csrc" / "rp_world.cpp", HERE / "csrc" / "rp_refsyms.cpp"]
 HEADERS = [HERE / "csrc" / "rp_kernels.h", HERE / "csrc" / "rp_step.h", HERE / "csrc" / "rp_math.h",
            HERE.parent / "include" / "rawphys.h"]
 

@@ -21,7 +21,6 @@ struct DevWorld {
     M4* model;                      // [nb]  model matrix of the current substep
     const int32_t* body_hull;       // [nb]
     const int32_t* body_tv_off;     // [nb]
-    const int32_t* body_tn_off;     // [nb]
     // hull instances (vertices scaled, local frame)
     const int32_t *hull_nv, *hull_nf, *hull_v_off, *hull_fn_off, *hull_topo;
     const V3* hull_v;
@@ -30,8 +29,8 @@ struct DevWorld {
     const int32_t* topo_pool;
     // transformed geometry
     V3* tv;
-    V3* tn;
-    const int2* xf;                 // [n_xf] (body, vertex) or (body, ~face)
+    const int2* xf;                 // [n_xf] (body, vertex)
+    int32_t* need;                  // [nb] body takes part in a live pair this frame
     // pairs
     int2* pairs;                    // [pair capacity] scene-local (i, j)
     int32_t* pair_count;            // [ns]

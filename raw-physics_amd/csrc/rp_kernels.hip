@@ -55,6 +55,7 @@ __global__ void k_broad(DevWorld w) {
     int b0 = w.scene_body_off[s], n = w.scene_body_off[s + 1] - b0;
     int base = w.scene_pair_base[s];
     int count = 0;
+    for (int i = lane; i < n; i += 64) w.need[b0 + i] = 0;
     for (int i = 0; i < n; ++i) {
         const Body& e1 = w.body[b0 + i];
         V3 x1 = e1.x;
@@ -175,7 +176,14 @@ __global__ void k_scatter_live(DevWorld w) {
     int s = blockIdx.x;
     int base = w.scene_pair_base[s];
     int g0 = w.live_prefix[s], n = w.live_count[s];
-    for (int m = threadIdx.x; m < n; m += blockDim.x) w.live_global[g0 + m] = make_int2(s, base + w.live_local[base + m]);
+    int b0 = w.scene_body_off[s];
+    for (int m = threadIdx.x; m < n; m += blockDim.x) {
+        int slot = base + w.live_local[base + m];
+        w.live_global[g0 + m] = make_int2(s, slot);
+        int2 p = w.pairs[slot];
+        w.need[b0 + p.x] = 1;  // same value from every writer
+        w.need[b0 + p.y] = 1;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -193,30 +201,24 @@ __global__ void k_integrate(DevWorld w, double h) {
     w.model[b] = model_matrix(e.x, e.q);
 }
 
-// collider_update (collider.cpp:409-430): vertices M*[v,1]/w, normals normalize(M3*n)
-__global__ void k_transform(DevWorld w) {
+// collider_update for the vertices (collider.cpp:409-421): M*[v,1], then
+// scaled by 1/w. Only bodies that take part in a live pair this frame.
+__global__ void k_transform(DevWorld w, int all) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= w.n_xf) return;
     int2 el = w.xf[t];
     int b = el.x;
+    if (!all && !w.need[b]) return;
     const M4& M = w.model[b];
     int hid = w.body_hull[b];
-    if (el.y >= 0) {
-        int i = el.y;
-        V3 v = w.hull_v[w.hull_v_off[hid] + i];
-        double x = v.x * M.m[0][0] + v.y * M.m[0][1] + v.z * M.m[0][2] + 1.0 * M.m[0][3];
-        double y = v.x * M.m[1][0] + v.y * M.m[1][1] + v.z * M.m[1][2] + 1.0 * M.m[1][3];
-        double z = v.x * M.m[2][0] + v.y * M.m[2][1] + v.z * M.m[2][2] + 1.0 * M.m[2][3];
-        double ww = v.x * M.m[3][0] + v.y * M.m[3][1] + v.z * M.m[3][2] + 1.0 * M.m[3][3];
-        double sc = 1.0 / ww;
-        w.tv[w.body_tv_off[b] + i] = v3(sc * x, sc * y, sc * z);
-    } else {
-        int f = ~el.y;
-        V3 n = w.hull_fn[w.hull_fn_off[hid] + f];
-        V3 r = v3(M.m[0][0] * n.x + M.m[0][1] * n.y + M.m[0][2] * n.z, M.m[1][0] * n.x + M.m[1][1] * n.y + M.m[1][2] * n.z,
-                  M.m[2][0] * n.x + M.m[2][1] * n.y + M.m[2][2] * n.z);
-        w.tn[w.body_tn_off[b] + f] = nrm(r);
-    }
+    int i = el.y;
+    V3 v = w.hull_v[w.hull_v_off[hid] + i];
+    double x = v.x * M.m[0][0] + v.y * M.m[0][1] + v.z * M.m[0][2] + 1.0 * M.m[0][3];
+    double y = v.x * M.m[1][0] + v.y * M.m[1][1] + v.z * M.m[1][2] + 1.0 * M.m[1][3];
+    double z = v.x * M.m[2][0] + v.y * M.m[2][1] + v.z * M.m[2][2] + 1.0 * M.m[2][3];
+    double ww = v.x * M.m[3][0] + v.y * M.m[3][1] + v.z * M.m[3][2] + 1.0 * M.m[3][3];
+    double sc = 1.0 / ww;
+    w.tv[w.body_tv_off[b] + i] = v3(sc * x, sc * y, sc * z);
 }
 
 __device__ __forceinline__ HullView hull_view(const DevWorld& w, int b) {
@@ -225,7 +227,12 @@ __device__ __forceinline__ HullView hull_view(const DevWorld& w, int b) {
     int tp = w.hull_topo[hid];
     const int32_t* tb = w.topo_base + 8 * tp;
     hv.tv = w.tv + w.body_tv_off[b];
-    hv.tn = w.tn + w.body_tn_off[b];
+    hv.fn = w.hull_fn + w.hull_fn_off[hid];
+    const M4& M = w.model[b];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) hv.R.m[i][j] = M.m[i][j];
     hv.nv = w.hull_nv[hid];
     hv.nf = w.hull_nf[hid];
     hv.fe_off = w.topo_pool + tb[0];
@@ -451,7 +458,7 @@ hipError_t launch_frame(const DevWorld& w, const FrameParams& fp, hipStream_t st
         t.end(st);
         if (fp.collisions) {
             t.begin(KC_TRANSFORM, st);
-            hipLaunchKernelGGL(k_transform, dim3(cdiv(w.n_xf, 256)), dim3(256), 0, st, w);
+            hipLaunchKernelGGL(k_transform, dim3(cdiv(w.n_xf, 256)), dim3(256), 0, st, w, 0);
             t.end(st);
             t.begin(KC_NARROW, st);
             hipLaunchKernelGGL(k_narrow, dim3(w.narrow_blocks), dim3(64), 0, st, w);
@@ -467,7 +474,7 @@ hipError_t launch_frame(const DevWorld& w, const FrameParams& fp, hipStream_t st
 hipError_t launch_pair_contacts(const DevWorld& w, int b1, int b2, double* out, int* count, uint32_t* err,
                                 hipStream_t st) {
     hipLaunchKernelGGL(k_model, dim3(cdiv(w.nb, 128)), dim3(128), 0, st, w);
-    hipLaunchKernelGGL(k_transform, dim3(cdiv(w.n_xf, 256)), dim3(256), 0, st, w);
+    hipLaunchKernelGGL(k_transform, dim3(cdiv(w.n_xf, 256)), dim3(256), 0, st, w, 1);
     hipLaunchKernelGGL(k_pair_contacts, dim3(1), dim3(1), 0, st, w, b1, b2, out, count, err);
     return hipGetLastError();
 }

@@ -41,13 +41,26 @@ struct ExtC {
     double limits[4];
 };
 
-// Hull topology and transformed geometry of one body.
+// Hull topology and geometry of one body at the current substep. Vertices
+// are pre-transformed; face normals are transformed when first needed, with
+// the reference's arithmetic (collider.cpp:426-428), so that only the few
+// faces the manifold visits are computed.
 struct HullView {
     const V3* tv;  // transformed vertices
-    const V3* tn;  // transformed face normals
+    const V3* fn;  // local face normals
+    M3 R;          // upper-left 3x3 of the model matrix T*R (util.cpp:60-74)
     int nv, nf;
     const int32_t *fe_off, *fe, *vf_off, *vf, *vn_off, *vn, *ff_off, *ff;
 };
+
+// gm_mat4_multiply_vec3(M, n, false) then normalize (collider.cpp:426-428)
+RP_HD V3 face_normal_world(const HullView& h, int f) {
+    V3 n = h.fn[f];
+    const M3& M = h.R;
+    V3 r = v3(M.m[0][0] * n.x + M.m[0][1] * n.y + M.m[0][2] * n.z, M.m[1][0] * n.x + M.m[1][1] * n.y + M.m[1][2] * n.z,
+              M.m[2][0] * n.x + M.m[2][1] * n.y + M.m[2][2] * n.z);
+    return nrm(r);
+}
 
 enum : uint32_t {
     ERR_GJK_SIMPLEX = 1u << 0,
@@ -73,10 +86,23 @@ constexpr int MAXC = 16;       // contacts kept per body pair
 // ---------------------------------------------------------------------------
 // Support mapping (support.cpp:5-39): first index wins on ties
 // ---------------------------------------------------------------------------
+// Vertices are loaded eight at a time so that their memory latencies
+// overlap; the comparisons still run in index order.
 RP_HD int support_index(const HullView& h, V3 d, uint32_t& err) {
     int sel = -1;
     double max_dot = -DBL_MAX;
-    for (int i = 0; i < h.nv; ++i) {
+    int i = 0;
+    for (; i + 8 <= h.nv; i += 8) {
+        V3 v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = h.tv[i + k];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            double dd = dot(v[k], d);
+            if (dd > max_dot) { sel = i + k; max_dot = dd; }
+        }
+    }
+    for (; i < h.nv; ++i) {
         double dd = dot(h.tv[i], d);
         if (dd > max_dot) { sel = i; max_dot = dd; }
     }
@@ -382,7 +408,7 @@ RP_HD int best_face(int sidx, const HullView& h, V3 n, uint32_t& err) {
     double max_proj = -DBL_MAX;
     int sel = -1;
     for (int i = h.vf_off[sidx]; i < h.vf_off[sidx + 1]; ++i) {
-        double proj = dot(h.tn[h.vf[i]], n);
+        double proj = dot(face_normal_world(h, h.vf[i]), n);
         if (proj > max_proj) { max_proj = proj; sel = h.vf[i]; }
     }
     if (sel < 0) { err |= ERR_SELECT; sel = 0; }
@@ -437,8 +463,8 @@ RP_HD int manifold(const HullView& h1, const HullView& h2, V3 normal, ContactPt*
         }
         if (!found) err |= ERR_SELECT;
     }
-    double d1 = dot(h1.tn[f1], normal);
-    double d2 = dot(h2.tn[f2], inormal);
+    double d1 = dot(face_normal_world(h1, f1), normal);
+    double d2 = dot(face_normal_world(h2, f2), inormal);
     double de = dot(edge_n, normal);
     int nc = 0;
     if (de > d1 + EPSILON && de > d2 + EPSILON) {
@@ -468,13 +494,13 @@ RP_HD int manifold(const HullView& h1, const HullView& h2, V3 normal, ContactPt*
         int nf = rh.ff[k];
         Plane p;
         p.point = rh.tv[rh.fe[rh.fe_off[nf]]];
-        p.normal = inv3(rh.tn[nf]);
+        p.normal = inv3(face_normal_world(rh, nf));
         n = clip_pass(in, n, p, outp, false, err);
         V3* t = in; in = outp; outp = t;
     }
     // remove the points above the reference plane (clipping.cpp:296-302)
     Plane rp;
-    rp.normal = inv3(rh.tn[rf]);
+    rp.normal = inv3(face_normal_world(rh, rf));
     rp.point = rh.tv[rh.fe[rh.fe_off[rf]]];
     n = clip_pass(in, n, rp, outp, true, err);
     // get_closest_point_polygon (clipping.cpp:115-119) and contact points (clipping.cpp:304-329)

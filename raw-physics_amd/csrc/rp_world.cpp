@@ -252,7 +252,7 @@ struct HostWorld {
 struct DeviceImage {
     DevWorld d;
     std::vector<void*> allocs;
-    std::vector<int32_t> body_tv_off, body_tn_off;  // host copies for inspection
+    std::vector<int32_t> body_tv_off;  // host copy for inspection
     bool ok = false;
 
     template <class T>
@@ -321,17 +321,13 @@ static bool build_image(const HostWorld& hw, DeviceImage& im) {
     }
     // per-body transformed geometry and the transform work list
     std::vector<int2> xf;
-    int64_t ntv = 0, ntn = 0;
+    int64_t ntv = 0;
     im.body_tv_off.resize(d.nb);
-    im.body_tn_off.resize(d.nb);
     for (int b = 0; b < d.nb; ++b) {
         const HostHull& h = hw.hulls[hw.body_hull[b]];
         im.body_tv_off[b] = (int32_t)ntv;
-        im.body_tn_off[b] = (int32_t)ntn;
         for (size_t i = 0; i < h.v.size(); ++i) xf.push_back(make_int2(b, (int)i));
-        for (size_t f = 0; f < h.fn.size(); ++f) xf.push_back(make_int2(b, ~(int)f));
         ntv += (int64_t)h.v.size();
-        ntn += (int64_t)h.fn.size();
     }
     d.n_xf = (int)xf.size();
     // pair and constraint capacities per scene
@@ -363,7 +359,6 @@ static bool build_image(const HostWorld& hw, DeviceImage& im) {
     ok &= im.upload(d.body_hull, hw.body_hull);
     const int32_t* p;
     ok &= im.upload(p, im.body_tv_off); d.body_tv_off = p;
-    ok &= im.upload(p, im.body_tn_off); d.body_tn_off = p;
     ok &= im.upload(d.hull_nv, hull_nv);
     ok &= im.upload(d.hull_nf, hull_nf);
     ok &= im.upload(d.hull_v_off, hull_v_off);
@@ -374,7 +369,7 @@ static bool build_image(const HostWorld& hw, DeviceImage& im) {
     ok &= im.upload(d.topo_base, topo_base);
     ok &= im.upload(d.topo_pool, topo_pool);
     ok &= im.alloc(d.tv, (size_t)ntv);
-    ok &= im.alloc(d.tn, (size_t)ntn);
+    ok &= im.alloc(d.need, d.nb);
     ok &= im.upload(d.xf, xf);
     ok &= im.alloc(d.pairs, (size_t)pcap);
     ok &= im.alloc(d.pair_count, d.ns);
