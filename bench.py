@@ -99,14 +99,30 @@ def op_counts(scenes_mod, n_scenes: int, frames: int):
         sim.step(1)
         pairs += lib.orc_world_last_narrow_pairs(sim.h)
         contacts += lib.orc_world_last_contacts(sim.h)
-    ops = np.zeros(8, dtype=np.uint64)
+    ops = np.zeros(16, dtype=np.uint64)
     lib.orc_opcount_get(ops.ctypes.data)
     sim.close()
-    names = ("frame", "integrate", "transform", "narrow", "position", "velocity_update", "velocity_solve")
+    names = ("frame", "integrate", "transform", "narrow", "position", "velocity_update", "velocity_solve", "setup",
+             "manifold")
     d = {n: int(ops[i]) for i, n in enumerate(names)}
     d["solve"] = d["position"] + d["velocity_update"] + d["velocity_solve"]
     return {"ops": d, "scenes": n_scenes, "frames": frames, "narrow_pairs": int(pairs), "contacts": int(contacts),
             "body_substeps": world.dynamic_bodies() * 20 * frames}
+
+
+def copy_calibration(nbytes: int):
+    """One hipMemcpy device to device of `nbytes` (the runtime's copy kernel
+    reads and writes nbytes each)."""
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy calibration: {nbytes} bytes device to device")
 
 
 def main():
@@ -122,6 +138,16 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel timing run")
     ap.add_argument("--op-scenes", type=int, default=8, help="scenes in the op-count sample")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tune-from", type=int, default=90, help="frame the autotuner's trials start from")
+    ap.add_argument("--tune-frames", type=int, default=6)
+    ap.add_argument("--narrow-lanes", type=int, default=0, help="fixed narrow schedule (lanes per pair); skips the autotuner")
+    ap.add_argument("--manifold-lanes", type=int, default=0, help="fixed manifold schedule (lanes per pair); skips the autotuner")
+    ap.add_argument("--solve-lanes", type=int, default=0, help="fixed solve schedule (lanes per scene); skips the autotuner")
+    ap.add_argument("--eager", action="store_true", help="launch each kernel eagerly instead of replaying frame graphs")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run one device-to-device copy of MIB MiB (a kernel of known traffic that scales "
+                         "the FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -134,6 +160,8 @@ def main():
     sc = pkg.scenes
     substeps, dt = 20, 1.0 / 60.0
 
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration << 20)
     t0 = time.perf_counter()
     world = sc.batched_uniform(args.scenes)
     log(f"built {args.scenes} scenes ({world.num_bodies} bodies) in {time.perf_counter() - t0:.1f} s")
@@ -141,6 +169,25 @@ def main():
     gw = pkg.World(world)
     log(f"uploaded world in {time.perf_counter() - t0:.1f} s")
     init = gw.state()
+
+    # schedule: the package's autotuner times each narrow-phase variant on a
+    # few frames from a mid-run state (bodies landed, contacts present)
+    tune = None
+    if args.eager:
+        gw.set_graphs(False)
+    if args.narrow_lanes:
+        gw.set_narrow_group(args.narrow_lanes)
+    if args.manifold_lanes:
+        gw.set_manifold_group(args.manifold_lanes)
+    if args.solve_lanes:
+        gw.set_solve_lanes(args.solve_lanes)
+    if not (args.no_autotune or args.narrow_lanes or args.manifold_lanes or args.solve_lanes):
+        gw.step(args.tune_from)
+        tune = gw.autotune(frames=args.tune_frames)
+        gw.set_state(init)
+        log(f"autotune (kernel ms over {args.tune_frames} frames): {tune} -> narrow {gw.narrow_group} lanes/pair, "
+            f"manifold {gw.manifold_group} lanes/pair, "
+            f"solve {gw.solve_lanes} lanes/scene")
 
     # warmup (first launches, code object load), then restore the initial state
     if args.warmup > 0:
@@ -155,6 +202,7 @@ def main():
     dyn = world.dynamic_bodies()
     value = dyn * substeps * args.steps / elapsed
     errs = gw.errors()
+    narrow_group, manifold_group, solve_lanes = gw.narrow_group, gw.manifold_group, gw.solve_lanes
     log(f"batched: {args.steps} frames in {elapsed:.3f} s -> {value:.4g} body-substeps/s; "
         f"error words nonzero in {int((errs != 0).sum())} scenes; EPA non-convergence {gw.epa_failures()}")
 
@@ -203,23 +251,30 @@ def main():
         if oc is not None:
             scale = args.scenes / oc["scenes"]
             kernels = {}
-            for cls in ("integrate", "transform", "narrow", "solve"):
+            for cls in ("integrate", "transform", "narrow", "manifold", "solve"):
                 ms, n = ktimes[cls]
                 flops = oc["ops"][cls] * scale
                 kernels[cls] = {"ms_total": ms, "launches": n, "avg_us": 1e3 * ms / max(n, 1),
                                 "algorithmic_flops": flops,
                                 "tflops": flops / (ms * 1e-3) / 1e12 if ms > 0 else None}
-            nar = kernels["narrow"]
-            roofline = {"bound": "fp64-valu", "kernel": "k_narrow (GJK + EPA + contact manifold)",
-                        "achieved": nar["tflops"], "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                        "frac": nar["tflops"] / FP64_PEAK_TFLOPS if nar["tflops"] else None,
+            # the dominant kernel: largest share of the frame
+            names = {"narrow": "k_gjk_epa (GJK + EPA per live pair)",
+                     "manifold": "k_manifold (contact manifold per colliding pair)",
+                     "solve": "k_solve (levelled position + velocity solve)"}
+            dom = max(names, key=lambda c: kernels[c]["ms_total"])
+            k = kernels[dom]
+            unit_ops = {"narrow": (oc["ops"]["narrow"], oc["narrow_pairs"], "live pair"),
+                        "manifold": (oc["ops"]["manifold"], oc["narrow_pairs"], "live pair"),
+                        "solve": (oc["ops"]["solve"], oc["body_substeps"], "body-substep")}[dom]
+            roofline = {"bound": "fp64-valu", "kernel": names[dom],
+                        "achieved": k["tflops"], "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": k["tflops"] / FP64_PEAK_TFLOPS if k["tflops"] else None,
                         "traffic": None,
-                        "flops_per_launch": nar["algorithmic_flops"] / max(nar["launches"], 1),
-                        "avg_launch_us": nar["avg_us"],
+                        "flops_per_launch": k["algorithmic_flops"] / max(k["launches"], 1),
+                        "avg_launch_us": k["avg_us"],
                         "op_sample": f"{oc['scenes']} scenes x {oc['frames']} frames counted by the oracle's "
                                      f"op-counting build, scaled x{scale:g} to {args.scenes} scenes; "
-                                     f"{oc['ops']['narrow'] / max(oc['narrow_pairs'], 1):.0f} FP64 ops per "
-                                     f"narrow-phase pair"}
+                                     f"{unit_ops[0] / max(unit_ops[1], 1):.0f} FP64 ops per {unit_ops[2]}"}
             log("roofline: " + json.dumps(roofline))
 
     out = {
@@ -237,7 +292,9 @@ def main():
         "data": "synthetic (seeded scenes)",
         "config": {"workload": "batched uniform: 4096 scenes x (floor + 8 cubes + 8 ico42), 600 frames, "
                                "dt=1/60, 20 substeps, 1 position iteration",
-                   "scenes": args.scenes, "dynamic_bodies": dyn, "frames": args.steps, "substeps": substeps},
+                   "scenes": args.scenes, "dynamic_bodies": dyn, "frames": args.steps, "substeps": substeps,
+                   "narrow_lanes_per_pair": narrow_group, "manifold_lanes_per_pair": manifold_group,
+                   "solve_lanes_per_scene": solve_lanes, "autotune_ms": tune},
         "plain": {"value": plain, "unit": "body-substeps/s",
                   "ratio_batched_over_plain": (value / plain) if plain else None},
         "roofline": roofline,

@@ -243,8 +243,33 @@ void rp_world_set_state(rp_world* w, const rp_state* in);
 uint32_t rp_world_error(rp_world* w, uint32_t scene);
 /* EPA runs that did not converge since creation (epa.cpp:233-236). */
 uint64_t rp_world_epa_failures(rp_world* w);
+/* GJK runs (gjk.cpp:350-379) that were ended early on a repeated loop state
+ * since creation; each returns what the full 100 iterations would. */
+uint64_t rp_world_gjk_cycle_exits(rp_world* w);
 /* The HIP stream the world's kernels run on (a hipStream_t). */
 void* rp_world_stream(rp_world* w);
+
+/* Narrow-phase schedules: lanes cooperating on one body pair in the GJK + EPA
+ * kernel (narrow group) and in the contact-manifold kernel (manifold group):
+ * 1 = one lane per pair over pre-transformed hulls; 4, 8 or 16 = a lane group
+ * per pair with hull vertices staged in LDS and split support searches. All
+ * schedules give identical results. Return 0, or -1 if the schedule does not
+ * fit this world (LDS). rp_world_create picks a narrow group of 8 where it
+ * fits and a manifold group of 1. */
+int32_t rp_world_set_narrow_group(rp_world* w, int32_t lanes_per_pair);
+int32_t rp_world_narrow_group(const rp_world* w);
+int32_t rp_world_set_manifold_group(rp_world* w, int32_t lanes_per_pair);
+int32_t rp_world_manifold_group(const rp_world* w);
+
+/* Solve schedule: lanes serving one scene in the solve kernel (64, 32 or 16,
+ * i.e. 1, 2 or 4 scenes per wave). Identical results. Returns 0 or -1. */
+int32_t rp_world_set_solve_lanes(rp_world* w, int32_t lanes_per_scene);
+int32_t rp_world_solve_lanes(const rp_world* w);
+
+/* Frame launches as a HIP graph (default on): one frame's kernels are
+ * captured once per step parameters and schedules and replayed per frame.
+ * Off: kernels are launched one by one. Identical results. */
+void rp_world_set_graphs(rp_world* w, int32_t on);
 
 /* Profiling mode: HIP events around every kernel launch of the step, read
  * after each frame (the frame loop then synchronizes once per frame).

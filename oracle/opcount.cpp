@@ -14,7 +14,7 @@
 #include <cstring>
 
 namespace {
-uint64_t g_ops[8];
+uint64_t g_ops[16];
 int g_phase = 7;
 }  // namespace
 
@@ -51,10 +51,10 @@ static inline real asin(real a) { ++g_ops[g_phase]; return real(std::asin(a.v));
 #include "rp_oracle.c"
 
 extern "C" {
-__attribute__((visibility("default"))) void orc_opcount_get(uint64_t out[8]) {
-    for (int i = 0; i < 8; ++i) out[i] = g_ops[i];
+__attribute__((visibility("default"))) void orc_opcount_get(uint64_t out[16]) {
+    for (int i = 0; i < 16; ++i) out[i] = g_ops[i];
 }
 __attribute__((visibility("default"))) void orc_opcount_reset(void) {
-    for (int i = 0; i < 8; ++i) g_ops[i] = 0;
+    for (int i = 0; i < 16; ++i) g_ops[i] = 0;
 }
 }

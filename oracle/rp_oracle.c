@@ -24,7 +24,8 @@ typedef double real;
 #define RD(x) (x)
 #define ORC_PHASE(p) ((void)0)
 #endif
-enum { PH_FRAME = 0, PH_INTEGRATE, PH_TRANSFORM, PH_NARROW, PH_POSITION, PH_VELOCITY_UPDATE, PH_VELOCITY_SOLVE, PH_SETUP };
+enum { PH_FRAME = 0, PH_INTEGRATE, PH_TRANSFORM, PH_NARROW, PH_POSITION, PH_VELOCITY_UPDATE, PH_VELOCITY_SOLVE, PH_SETUP,
+       PH_MANIFOLD, PH_COUNT };
 
 /* ------------------------------------------------------------------ */
 /* Growable arrays with the reference's swap-remove semantics          */
@@ -974,7 +975,9 @@ static void pair_contacts(const hull* h1, const hull* h2, contactarr* out, uint3
     if (gjk(h1, h2, &s, err)) {
         v3 n; real pen;
         if (!epa(h1, h2, &s, &n, &pen, err)) { ++*epa_fail; return; }
+        ORC_PHASE(PH_MANIFOLD);
         manifold(h1, h2, n, out, err);
+        ORC_PHASE(PH_NARROW);
     }
 }
 

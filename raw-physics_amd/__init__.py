@@ -41,6 +41,7 @@ def library():
         "rp_world_set_state": (None, [_P, ctypes.POINTER(scenes.StateC)]),
         "rp_world_error": (u32, [_P, u32]),
         "rp_world_epa_failures": (u64, [_P]),
+        "rp_world_gjk_cycle_exits": (u64, [_P]),
         "rp_world_stream": (_P, [_P]),
         "rp_world_body_info": (None, [_P, _P, _P, _P, _P]),
         "rp_world_broad_pairs": (u32, [_P, u32, _P, u32]),
@@ -52,6 +53,13 @@ def library():
         "physics_simulate": (None, [_P, f64]),
         "rp_fp64_selftest": (u32, [_P, _P, _P, u64]),
         "rp_world_set_profiling": (None, [_P, i32]),
+        "rp_world_set_narrow_group": (i32, [_P, i32]),
+        "rp_world_narrow_group": (i32, [_P]),
+        "rp_world_set_manifold_group": (i32, [_P, i32]),
+        "rp_world_manifold_group": (i32, [_P]),
+        "rp_world_set_solve_lanes": (i32, [_P, i32]),
+        "rp_world_solve_lanes": (i32, [_P]),
+        "rp_world_set_graphs": (None, [_P, i32]),
         "rp_world_kernel_times": (None, [_P, _P, _P]),
     }
     for name, (res, args) in sig.items():
@@ -117,7 +125,75 @@ class World:
     def epa_failures(self) -> int:
         return int(self.lib.rp_world_epa_failures(self.h))
 
-    KERNEL_CLASSES = ("frame", "integrate", "transform", "narrow", "solve")
+    def gjk_cycle_exits(self) -> int:
+        return int(self.lib.rp_world_gjk_cycle_exits(self.h))
+
+    KERNEL_CLASSES = ("frame", "integrate", "transform", "narrow", "solve", "manifold")
+
+    def set_narrow_group(self, lanes_per_pair: int):
+        """GJK + EPA schedule: 1, 4, 8 or 16 lanes per body pair (same results)."""
+        if self.lib.rp_world_set_narrow_group(self.h, lanes_per_pair) != 0:
+            raise ValueError(f"narrow-phase schedule {lanes_per_pair} does not fit this world")
+
+    def set_manifold_group(self, lanes_per_pair: int):
+        """Contact-manifold schedule: 1, 4, 8 or 16 lanes per body pair (same results)."""
+        if self.lib.rp_world_set_manifold_group(self.h, lanes_per_pair) != 0:
+            raise ValueError(f"manifold schedule {lanes_per_pair} does not fit this world")
+
+    def autotune(self, frames: int = 6, narrow=(1, 4, 8, 16), manifold=(1, 4, 8, 16), solve=(64, 32, 16),
+                 **step_kw) -> dict:
+        """Picks the GJK + EPA, manifold and solve schedules from per-kernel
+        times: each trial runs `frames` frames from the current state in
+        profiling mode with one candidate of each kind, and the state is
+        restored after each trial. A one-lane-per-pair schedule is charged the
+        vertex transform it needs. All schedules give identical results.
+        Returns {kind: {candidate: ms}}."""
+        start = self.state()
+        narrow = [g for g in narrow if self.lib.rp_world_set_narrow_group(self.h, g) == 0]
+        manifold = [g for g in manifold if self.lib.rp_world_set_manifold_group(self.h, g) == 0]
+        solve = [n for n in solve if self.lib.rp_world_set_solve_lanes(self.h, n) == 0]
+        out = {"narrow": {}, "manifold": {}, "solve": {}}
+        trials = max(len(narrow), len(manifold), len(solve))
+        for t in range(trials):
+            g, m, n = narrow[t % len(narrow)], manifold[t % len(manifold)], solve[t % len(solve)]
+            self.set_narrow_group(g)
+            self.set_manifold_group(m)
+            self.set_solve_lanes(n)
+            self.step(1, **step_kw)  # first launches of these variants
+            self.set_state(start)
+            self.set_profiling(True)
+            self.step(frames, **step_kw)
+            kt = self.kernel_times()
+            self.set_profiling(False)
+            self.set_state(start)
+            xf = kt["transform"][0]
+            out["narrow"].setdefault(g, kt["narrow"][0] + (xf if g == 1 else 0.0))
+            out["manifold"].setdefault(m, kt["manifold"][0] + (xf if m == 1 else 0.0))
+            out["solve"].setdefault(n, kt["solve"][0])
+        self.set_narrow_group(min(out["narrow"], key=out["narrow"].get))
+        self.set_manifold_group(min(out["manifold"], key=out["manifold"].get))
+        self.set_solve_lanes(min(out["solve"], key=out["solve"].get))
+        return out
+
+    def set_graphs(self, on: bool):
+        """Replay each frame as a captured HIP graph (default) or launch eagerly."""
+        self.lib.rp_world_set_graphs(self.h, 1 if on else 0)
+
+    def set_solve_lanes(self, lanes_per_scene: int):
+        if self.lib.rp_world_set_solve_lanes(self.h, lanes_per_scene) != 0:
+            raise ValueError(f"solve schedule {lanes_per_scene} lanes per scene is not available")
+
+    @property
+    def solve_lanes(self) -> int:
+        return int(self.lib.rp_world_solve_lanes(self.h))
+
+    @property
+    def narrow_group(self) -> int:
+        return int(self.lib.rp_world_narrow_group(self.h))
+
+    @property
+    def manifold_group(self) -> int:
+        return int(self.lib.rp_world_manifold_group(self.h))
 
     def set_profiling(self, on: bool):
         self.lib.rp_world_set_profiling(self.h, 1 if on else 0)

@@ -6,16 +6,22 @@
 
 namespace rp {
 
-enum { STAT_EPA_FAIL = 0, STAT_COUNT = 4 };
+enum { STAT_EPA_FAIL = 0, STAT_GJK_CYCLE = 1, STAT_COUNT = 4 };
+enum { CNT_MANIFOLD = 0, CNT_COUNT = 4 };
 
 // All device pointers of one world. Passed by value to every kernel.
 struct DevWorld {
-    int ns, nb, n_xf, narrow_blocks;
+    int ns, nb, n_xf, narrow_blocks, n_ext;
+    int narrow_group;               // lanes per pair in GJK + EPA: 1 (lane per pair) or 4, 8, 16
+    int manifold_group;             // lanes per pair in the contact manifold: 1, 4, 8, 16
+    int manifold_blocks;
+    int stage_cap;                  // LDS vertices per pair: 2 x the largest hull
+    int solve_lanes;                // lanes per scene in the solve: 64, 32 or 16
     // scenes
     const int32_t* scene_body_off;  // [ns+1]
     const int32_t* scene_pair_base; // [ns+1]  candidate pair capacity prefix: n(n-1)/2 per scene
     const int32_t* scene_ext_off;   // [ns+1]
-    const int32_t* scene_con_base;  // [ns+1]  constraint capacity prefix
+    const int32_t* scene_item_base; // [ns+1]  solve-item capacity prefix: external constraints + pair capacity
     // bodies
     Body* body;                     // [nb]
     M4* model;                      // [nb]  model matrix of the current substep
@@ -39,13 +45,20 @@ struct DevWorld {
     int32_t* live_prefix;           // [ns+1]
     int2* live_global;              // [pair capacity] (scene, pair slot)
     int32_t* ncontact;              // [pair capacity]
+    V3* pair_normal;                // [pair capacity] EPA normal of the pairs queued for the manifold
+    int32_t* mlist;                 // [pair capacity] manifold queue: live-pair slots
+    int32_t* counters;              // [CNT_COUNT] queue lengths of the current substep
     Contact* contacts;              // [pair capacity * MAXC]
     // external constraints
     const ExtC* ext;                // [n_ext]
     double* ext_lambda;             // [3 n_ext]
     // solver scratch
-    int32_t *con_ref, *con_level, *con_order, *con_lstart;
-    int32_t* body_last;             // [nb]
+    int32_t* item_level;            // [items]       level of each solve item (frame)
+    int32_t* item_order;            // [items]       items grouped by level
+    int32_t* item_lstart;           // [items + ns]  level starts per scene
+    int32_t* item_cursor;           // [items + ns]  fill cursors of the sort
+    int32_t* scene_nlevels;         // [ns]
+    int32_t* body_last;             // [nb]          last level per body (large scenes)
     int32_t *uf_parent, *uf_flag;   // [nb]
     // diagnostics
     uint32_t* scene_err;            // [ns]
@@ -60,7 +73,7 @@ struct FrameParams {
 };
 
 // Kernel classes timed by the profiling mode.
-enum { KC_FRAME = 0, KC_INTEGRATE, KC_TRANSFORM, KC_NARROW, KC_SOLVE, KC_COUNT = 8 };
+enum { KC_FRAME = 0, KC_INTEGRATE, KC_TRANSFORM, KC_NARROW, KC_SOLVE, KC_MANIFOLD, KC_COUNT = 8 };
 
 // Records a pair of HIP events around each timed launch (profiling mode
 // only); the host reads the elapsed times after the frame completes.
@@ -93,6 +106,9 @@ hipError_t launch_frame(const DevWorld& w, const FrameParams& fp, hipStream_t st
 hipError_t launch_pair_contacts(const DevWorld& w, int b1, int b2, double* out, int* count, uint32_t* err,
                                 hipStream_t st);
 hipError_t launch_broad(const DevWorld& w, hipStream_t st);
+// Lanes per pair the narrow phase can use for this world (LDS permitting).
+bool narrow_group_supported(const DevWorld& w, int g);
+int narrow_blocks_for(const DevWorld& w, int g, int64_t pair_capacity);
 hipError_t launch_fp64_selftest(const double* a, const double* b, double* out, int64_t n, hipStream_t st);
 
 }  // namespace rp

@@ -20,6 +20,8 @@
 // any solve (pbd_base_constraints.cpp:103-133, pbd.cpp:702-711), so sharing
 // one does not order two constraints. Every body therefore sees the same
 // sequence of updates as in the sequential loop and results are bit-identical.
+#include <algorithm>
+
 #include "rp_kernels.h"
 
 namespace rp {
@@ -189,6 +191,7 @@ __global__ void k_scatter_live(DevWorld w) {
 // ---------------------------------------------------------------------------
 __global__ void k_integrate(DevWorld w, double h) {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b == 0) w.counters[CNT_MANIFOLD] = 0;  // the substep's manifold queue starts empty
     if (b >= w.nb) return;
     Body e = w.body[b];
     integrate(e, h);
@@ -201,24 +204,27 @@ __global__ void k_integrate(DevWorld w, double h) {
     w.model[b] = model_matrix(e.x, e.q);
 }
 
-// collider_update for the vertices (collider.cpp:409-421): M*[v,1], then
-// scaled by 1/w. Only bodies that take part in a live pair this frame.
+// collider_update for one vertex (collider.cpp:413-421): gm_mat4_multiply_vec4
+// of M with [v, 1] (gm.h:405-413), then scaled by 1/w.
+__device__ __forceinline__ V3 transform_vertex(const M4& M, V3 v) {
+    double x = v.x * M.m[0][0] + v.y * M.m[0][1] + v.z * M.m[0][2] + 1.0 * M.m[0][3];
+    double y = v.x * M.m[1][0] + v.y * M.m[1][1] + v.z * M.m[1][2] + 1.0 * M.m[1][3];
+    double z = v.x * M.m[2][0] + v.y * M.m[2][1] + v.z * M.m[2][2] + 1.0 * M.m[2][3];
+    double ww = v.x * M.m[3][0] + v.y * M.m[3][1] + v.z * M.m[3][2] + 1.0 * M.m[3][3];
+    double sc = 1.0 / ww;
+    return v3(sc * x, sc * y, sc * z);
+}
+
+// Transformed vertices of the bodies that take part in a live pair this
+// frame (lane-per-pair narrow phase and the inspection path).
 __global__ void k_transform(DevWorld w, int all) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= w.n_xf) return;
     int2 el = w.xf[t];
     int b = el.x;
     if (!all && !w.need[b]) return;
-    const M4& M = w.model[b];
     int hid = w.body_hull[b];
-    int i = el.y;
-    V3 v = w.hull_v[w.hull_v_off[hid] + i];
-    double x = v.x * M.m[0][0] + v.y * M.m[0][1] + v.z * M.m[0][2] + 1.0 * M.m[0][3];
-    double y = v.x * M.m[1][0] + v.y * M.m[1][1] + v.z * M.m[1][2] + 1.0 * M.m[1][3];
-    double z = v.x * M.m[2][0] + v.y * M.m[2][1] + v.z * M.m[2][2] + 1.0 * M.m[2][3];
-    double ww = v.x * M.m[3][0] + v.y * M.m[3][1] + v.z * M.m[3][2] + 1.0 * M.m[3][3];
-    double sc = 1.0 / ww;
-    w.tv[w.body_tv_off[b] + i] = v3(sc * x, sc * y, sc * z);
+    w.tv[w.body_tv_off[b] + el.y] = transform_vertex(w.model[b], w.hull_v[w.hull_v_off[hid] + el.y]);
 }
 
 __device__ __forceinline__ HullView hull_view(const DevWorld& w, int b) {
@@ -246,120 +252,263 @@ __device__ __forceinline__ HullView hull_view(const DevWorld& w, int b) {
     return hv;
 }
 
-// Narrow phase over the live pairs of all scenes (pbd.cpp:583-612). Each
-// lane owns one pair; results go to the pair's slot in the live list, which
-// is in scene then pair order.
-__global__ void __launch_bounds__(64) k_narrow(DevWorld w) {
-    int total = w.live_prefix[w.ns];
-    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += gridDim.x * blockDim.x) {
-        int2 sp = w.live_global[g];
-        int s = sp.x;
-        int2 p = w.pairs[sp.y];
-        int b0 = w.scene_body_off[s];
-        int b1 = b0 + p.x, b2 = b0 + p.y;
-        HullView h1 = hull_view(w, b1), h2 = hull_view(w, b2);
-        uint32_t err = 0;
-        int epa_fail = 0;
-        ContactPt cts[MAXC];
-        int nc = pair_contacts(h1, h2, cts, err, epa_fail);
-        const Body& e1 = w.body[b1];
-        const Body& e2 = w.body[b2];
-        Contact* out = w.contacts + (size_t)g * MAXC;
-        for (int k = 0; k < nc; ++k) contact_to_constraint(e1, e2, cts[k], out[k]);
-        w.ncontact[g] = nc;
-        if (err) atomicOr(&w.scene_err[s], err);
-        if (epa_fail) atomicAdd(&w.stats[STAT_EPA_FAIL], 1ull);
+// Narrow phase over the live pairs of all scenes (pbd.cpp:583-612), in two
+// kernels: GJK + EPA for every live pair, then the contact manifold for the
+// pairs whose EPA converged (collider.cpp:523-558). Both take G lanes per
+// pair: G = 1 is one lane per pair over the pre-transformed vertices
+// (k_transform); G = 4, 8, 16 put 64/G pairs in a wave, transform both
+// hulls' vertices into LDS and split every support search across the group
+// (support_index<G>); the rest runs identically on the group's lanes and
+// lane 0 of the group writes the results. Results go to the pair's slot g in
+// the live list (scene, then pair order), so the queue order between the
+// kernels does not matter.
+__device__ __forceinline__ void group_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Hull views of live pair g; for G > 1 the group's lanes transform both
+// hulls' vertices into the group's LDS stage `st`.
+template <int G>
+__device__ __forceinline__ void pair_views(const DevWorld& w, int g, V3* st, int gl, int& s, int& b1, int& b2,
+                                           HullView& h1, HullView& h2) {
+    int2 sp = w.live_global[g];
+    s = sp.x;
+    int2 p = w.pairs[sp.y];
+    int b0 = w.scene_body_off[s];
+    b1 = b0 + p.x;
+    b2 = b0 + p.y;
+    h1 = hull_view(w, b1);
+    h2 = hull_view(w, b2);
+    if constexpr (G > 1) {
+        const V3* lv1 = w.hull_v + w.hull_v_off[w.body_hull[b1]];
+        const V3* lv2 = w.hull_v + w.hull_v_off[w.body_hull[b2]];
+        for (int i = gl; i < h1.nv; i += G) st[i] = transform_vertex(w.model[b1], lv1[i]);
+        for (int i = gl; i < h2.nv; i += G) st[h1.nv + i] = transform_vertex(w.model[b2], lv2[i]);
+        group_lds_sync();
+        h1.tv = st;
+        h2.tv = st + h1.nv;
     }
 }
 
-// Solve of one scene: position Gauss-Seidel (pbd.cpp:615-620), velocity
-// update (pbd.cpp:623-643), velocity solve (pbd.cpp:646-711).
-__global__ void __launch_bounds__(64) k_solve(DevWorld w, double h, int iters, int collisions) {
+template <int G>
+__global__ void __launch_bounds__(64) k_gjk_epa(DevWorld w) {
+    extern __shared__ V3 rp_stage[];
+    constexpr int NG = 64 / G;
+    int grp = threadIdx.x / G, gl = threadIdx.x & (G - 1);
+    V3* st = rp_stage + grp * w.stage_cap;
+    int total = w.live_prefix[w.ns];
+    for (int g = blockIdx.x * NG + grp; g < total; g += gridDim.x * NG) {
+        int s, b1, b2;
+        HullView h1, h2;
+        pair_views<G>(w, g, st, gl, s, b1, b2, h1, h2);
+        uint32_t err = 0;
+        bool cycled = false;
+        Simplex sx;
+        V3 n = v3(0.0, 0.0, 0.0);
+        double pen;
+        int r = 0;
+        bool hit = gjk<G>(h1, h2, sx, err, cycled);
+        if (hit) r = epa<G>(h1, h2, sx, n, pen, err);
+        if (gl == 0) {
+            if (r == 1) {
+                w.pair_normal[g] = n;
+                w.mlist[atomicAdd(&w.counters[CNT_MANIFOLD], 1)] = g;
+            } else {
+                w.ncontact[g] = 0;
+            }
+            if (err) atomicOr(&w.scene_err[s], err);
+            if (hit && r != 1) atomicAdd(&w.stats[STAT_EPA_FAIL], 1ull);  // epa.cpp:233-236
+            if (cycled) atomicAdd(&w.stats[STAT_GJK_CYCLE], 1ull);
+        }
+        if constexpr (G > 1) group_lds_sync();
+    }
+}
+
+template <int G>
+__global__ void __launch_bounds__(64) k_manifold(DevWorld w) {
+    extern __shared__ V3 rp_stage[];
+    constexpr int NG = 64 / G;
+    int grp = threadIdx.x / G, gl = threadIdx.x & (G - 1);
+    V3* st = rp_stage + grp * w.stage_cap;
+    int total = w.counters[CNT_MANIFOLD];
+    for (int k = blockIdx.x * NG + grp; k < total; k += gridDim.x * NG) {
+        int g = w.mlist[k];
+        int s, b1, b2;
+        HullView h1, h2;
+        pair_views<G>(w, g, st, gl, s, b1, b2, h1, h2);
+        uint32_t err = 0;
+        ContactPt cts[MAXC];
+        int nc = manifold<G>(h1, h2, w.pair_normal[g], cts, err);
+        if (gl == 0) {
+            const Body& e1 = w.body[b1];
+            const Body& e2 = w.body[b2];
+            Contact* out = w.contacts + (size_t)g * MAXC;
+            for (int c = 0; c < nc; ++c) contact_to_constraint(e1, e2, cts[c], out[c]);
+            w.ncontact[g] = nc;
+            if (err) atomicOr(&w.scene_err[s], err);
+        }
+        if constexpr (G > 1) group_lds_sync();
+    }
+}
+
+// Dependency levels of one frame's solve items (wave/scene). An item is one
+// external constraint or the contacts of one live pair (solved in order by
+// one lane). The reference's sequence is: external constraints, then the
+// contacts of each pair in pair order (pbd.cpp:580-620). An item's level is
+// one above the last level of either of its non-fixed bodies, so items that
+// share a writable body keep their order and items of one level touch
+// disjoint writable bodies. Pairs are fixed for the frame (the broad phase
+// and activity run once per frame, pbd.cpp:474-506), so levels are too.
+constexpr int LEV_CHUNK = 512;
+constexpr int LEV_LDS_BODIES = 1024;
+
+__global__ void __launch_bounds__(64) k_levels(DevWorld w, int collisions) {
     int s = blockIdx.x;
     int lane = threadIdx.x;
     int b0 = w.scene_body_off[s], n = w.scene_body_off[s + 1] - b0;
-    int e0 = w.scene_ext_off[s], e1x = w.scene_ext_off[s + 1];
-    int next = e1x - e0;
-    int g0 = collisions ? w.live_prefix[s] : 0;
-    int g1 = collisions ? w.live_prefix[s] + w.live_count[s] : 0;
-    int cb = w.scene_con_base[s];
-    int ccap = w.scene_con_base[s + 1] - cb;
-    int* cref = w.con_ref + cb;     // constraint sequence
-    int* clev = w.con_level + cb;   // level of each constraint
-    int* order = w.con_order + cb;  // constraints sorted by level
-    int* lstart = w.con_lstart + cb + s;  // level starts (ccap + 1 entries)
-    int* last = w.body_last + b0;
-    __shared__ int sh_K, sh_L;
-    uint32_t err = 0;
-
-    // multipliers of the external constraints start at zero (pbd.cpp:426-462)
-    for (int k = e0 + lane; k < e1x; k += 64) {
-        w.ext_lambda[3 * k] = 0.0;
-        w.ext_lambda[3 * k + 1] = 0.0;
-        w.ext_lambda[3 * k + 2] = 0.0;
-    }
+    int e0 = w.scene_ext_off[s], next = w.scene_ext_off[s + 1] - e0;
+    int g0 = w.live_prefix[s];
+    int np = collisions ? w.live_count[s] : 0;
+    int K = next + np;
+    int ib = w.scene_item_base[s];
+    int* lev = w.item_level + ib;
+    int* order = w.item_order + ib;
+    int* ls = w.item_lstart + ib + s;
+    __shared__ int sb1[LEV_CHUNK], sb2[LEV_CHUNK];
+    __shared__ int slast[LEV_LDS_BODIES];
+    __shared__ int sL;
+    bool lds_last = n <= LEV_LDS_BODIES;
+    int* last = lds_last ? slast : (w.body_last + b0);
     for (int i = lane; i < n; i += 64) last[i] = 0;
+    if (lane == 0) sL = 0;
+    __syncthreads();
+    for (int c0 = 0; c0 < K; c0 += LEV_CHUNK) {
+        int cn = min(LEV_CHUNK, K - c0);
+        for (int k = lane; k < cn; k += 64) {
+            int t = c0 + k, i1, i2;
+            if (t < next) {
+                i1 = w.ext[e0 + t].b1 - b0;
+                i2 = w.ext[e0 + t].b2 - b0;
+            } else {
+                int2 p = w.pairs[w.live_global[g0 + t - next].y];
+                i1 = p.x;
+                i2 = p.y;
+            }
+            sb1[k] = w.body[b0 + i1].fixed ? -1 : i1;  // fixed bodies never order items
+            sb2[k] = w.body[b0 + i2].fixed ? -1 : i2;
+        }
+        __syncthreads();
+        if (lane == 0) {
+            int L = sL;
+            for (int k = 0; k < cn; ++k) {
+                int i1 = sb1[k], i2 = sb2[k];
+                int lv = 0;
+                if (i1 >= 0) lv = last[i1];
+                if (i2 >= 0) lv = max(lv, last[i2]);
+                ++lv;
+                if (i1 >= 0) last[i1] = lv;
+                if (i2 >= 0) last[i2] = lv;
+                lev[c0 + k] = lv - 1;
+                L = max(L, lv);
+            }
+            sL = L;
+        }
+        __syncthreads();
+    }
+    int L = sL;
+    // counting sort by level; the order inside a level does not matter (its
+    // items touch disjoint writable bodies)
+    int* cur = w.item_cursor + ib + s;
+    for (int l = lane; l <= L; l += 64) ls[l] = 0;
+    __syncthreads();
+    for (int t = lane; t < K; t += 64) atomicAdd(&ls[lev[t] + 1], 1);
     __syncthreads();
     if (lane == 0) {
-        // constraint sequence: external constraints, then contacts in pair order
-        int K = 0;
-        bool overflow = false;
-        for (int k = 0; k < next; ++k) {
-            if (K < ccap) cref[K++] = ~(e0 + k); else overflow = true;
-        }
-        for (int g = g0; g < g1; ++g) {
-            int nc = w.ncontact[g];
-            for (int c = 0; c < nc; ++c) {
-                if (K < ccap) cref[K++] = g * MAXC + c; else overflow = true;
-            }
-        }
-        if (overflow) err |= ERR_SOLVER_CAPACITY;
-        // levels: one more than the last level of either writable body
-        int L = 0;
-        for (int t = 0; t < K; ++t) {
-            int r = cref[t];
-            int i1, i2;
-            if (r < 0) { i1 = w.ext[~r].b1 - b0; i2 = w.ext[~r].b2 - b0; }
-            else { int2 p = w.pairs[w.live_global[r / MAXC].y]; i1 = p.x; i2 = p.y; }
-            bool f1 = w.body[b0 + i1].fixed, f2 = w.body[b0 + i2].fixed;
-            int lv = 0;
-            if (!f1) lv = last[i1];
-            if (!f2) lv = max(lv, last[i2]);
-            ++lv;
-            if (!f1) last[i1] = lv;
-            if (!f2) last[i2] = lv;
-            clev[t] = lv - 1;
-            L = max(L, lv);
-        }
-        // counting sort by level (stable)
-        for (int l = 0; l <= L; ++l) lstart[l] = 0;
-        for (int t = 0; t < K; ++t) ++lstart[clev[t] + 1];
-        for (int l = 0; l < L; ++l) lstart[l + 1] += lstart[l];
-        for (int t = 0; t < K; ++t) order[lstart[clev[t]]++] = t;
-        for (int l = L; l > 0; --l) lstart[l] = lstart[l - 1];
-        lstart[0] = 0;
-        sh_K = K;
-        sh_L = L;
+        for (int l = 0; l < L; ++l) ls[l + 1] += ls[l];
+        w.scene_nlevels[s] = L;
     }
     __syncthreads();
-    int K = sh_K, L = sh_L;
+    for (int l = lane; l <= L; l += 64) cur[l] = ls[l];
+    __syncthreads();
+    for (int t = lane; t < K; t += 64) {
+        int pos = atomicAdd(&cur[lev[t]], 1);
+        order[pos] = (t < next) ? ~(e0 + t) : (g0 + t - next);
+    }
+}
 
+// Loads of the body fields each solve reads (the rest of Body is unused).
+__device__ __forceinline__ void load_pos_body(const Body& g, Body& b) {
+    b.x = g.x; b.q = g.q; b.px = g.px; b.pq = g.pq;
+    b.inv_mass = g.inv_mass; b.Iinv = g.Iinv; b.mu_s = g.mu_s; b.fixed = g.fixed;
+}
+__device__ __forceinline__ void load_vel_body(const Body& g, Body& b) {
+    b.q = g.q; b.v = g.v; b.w = g.w; b.pv = g.pv; b.pw = g.pw;
+    b.inv_mass = g.inv_mass; b.Iinv = g.Iinv; b.mu_d = g.mu_d; b.e = g.e; b.fixed = g.fixed;
+}
+
+// Solve of one scene and substep: position Gauss-Seidel (pbd.cpp:615-620),
+// velocity update (pbd.cpp:623-643), velocity solve (pbd.cpp:646-711), level
+// by level over the frame's items. LPS lanes serve one scene, so a wave
+// solves 64/LPS scenes side by side (small scenes leave most of a 64-lane
+// wave idle otherwise).
+template <bool kExt, int LPS>
+__global__ void __launch_bounds__(64) k_solve(DevWorld w, double h, int iters, int collisions) {
+    constexpr int SPW = 64 / LPS;
+    int s = blockIdx.x * SPW + (int)threadIdx.x / LPS;
+    int lane = (int)threadIdx.x % LPS;
+    bool valid = s < w.ns;
+    int b0 = 0, n = 0, e0 = 0, e1x = 0, L = 0;
+    const int* order = nullptr;
+    const int* ls = nullptr;
+    if (valid) {
+        b0 = w.scene_body_off[s];
+        n = w.scene_body_off[s + 1] - b0;
+        e0 = w.scene_ext_off[s];
+        e1x = w.scene_ext_off[s + 1];
+        int ib = w.scene_item_base[s];
+        order = w.item_order + ib;
+        ls = w.item_lstart + ib + s;
+        L = w.scene_nlevels[s];
+    }
+    int Lmax = L;  // levels of the wave's busiest scene: every lane joins every barrier
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) Lmax = max(Lmax, __shfl_xor(Lmax, off));
+    uint32_t err = 0;
+    (void)collisions;
+
+    // multipliers of the external constraints start at zero (pbd.cpp:426-462)
+    if constexpr (kExt) {
+        for (int k = e0 + lane; k < e1x; k += LPS) {
+            w.ext_lambda[3 * k] = 0.0;
+            w.ext_lambda[3 * k + 1] = 0.0;
+            w.ext_lambda[3 * k + 2] = 0.0;
+        }
+    }
+    __syncthreads();
     for (int it = 0; it < iters; ++it) {
-        for (int l = 0; l < L; ++l) {
-            for (int idx = lstart[l] + lane; idx < lstart[l + 1]; idx += 64) {
-                int r = cref[order[idx]];
+        for (int l = 0; l < Lmax; ++l) {
+            int i0 = (l < L) ? ls[l] : 0, i1 = (l < L) ? ls[l + 1] : 0;
+            for (int idx = i0 + lane; idx < i1; idx += LPS) {
+                int item = order[idx];
                 int bi1, bi2;
-                if (r < 0) { bi1 = w.ext[~r].b1; bi2 = w.ext[~r].b2; }
-                else { int2 p = w.pairs[w.live_global[r / MAXC].y]; bi1 = b0 + p.x; bi2 = b0 + p.y; }
-                Body B1 = w.body[bi1];
-                Body B2 = w.body[bi2];
-                if (r < 0) {
-                    solve_external(w.ext[~r], w.ext_lambda + 3 * (~r), B1, B2, h, err);
+                if (item < 0) { bi1 = w.ext[~item].b1; bi2 = w.ext[~item].b2; }
+                else { int2 p = w.pairs[w.live_global[item].y]; bi1 = b0 + p.x; bi2 = b0 + p.y; }
+                Body B1, B2;
+                load_pos_body(w.body[bi1], B1);
+                load_pos_body(w.body[bi2], B2);
+                if (item < 0) {
+                    if constexpr (kExt) solve_external(w.ext[~item], w.ext_lambda + 3 * (~item), B1, B2, h, err);
                 } else {
-                    Contact c = w.contacts[r];
-                    solve_collision(c, B1, B2, h, err);
-                    w.contacts[r].lambda_n = c.lambda_n;
-                    w.contacts[r].lambda_t = c.lambda_t;
+                    int nc = w.ncontact[item];
+                    Contact* cs = w.contacts + (size_t)item * MAXC;
+                    for (int c = 0; c < nc; ++c) {
+                        Contact ct = cs[c];
+                        solve_collision(ct, B1, B2, h, err);
+                        cs[c].lambda_n = ct.lambda_n;
+                        cs[c].lambda_t = ct.lambda_t;
+                    }
                 }
                 if (!B1.fixed) { w.body[bi1].x = B1.x; w.body[bi1].q = B1.q; }
                 if (!B2.fixed) { w.body[bi2].x = B2.x; w.body[bi2].q = B2.q; }
@@ -368,34 +517,52 @@ __global__ void __launch_bounds__(64) k_solve(DevWorld w, double h, int iters, i
         }
     }
     // velocity update
-    for (int i = lane; i < n; i += 64) {
-        Body e = w.body[b0 + i];
-        if (e.fixed || !e.active) continue;
+    for (int i = lane; i < n; i += LPS) {
+        Body& g = w.body[b0 + i];
+        if (g.fixed || !g.active) continue;
+        Body e;
+        e.x = g.x; e.px = g.px; e.q = g.q; e.pq = g.pq; e.v = g.v; e.w = g.w;
+        e.fixed = 0; e.active = 1;
         velocity_update(e, h);
-        w.body[b0 + i].pv = e.pv;
-        w.body[b0 + i].pw = e.pw;
-        w.body[b0 + i].v = e.v;
-        w.body[b0 + i].w = e.w;
+        g.pv = e.pv; g.pw = e.pw; g.v = e.v; g.w = e.w;
     }
     __syncthreads();
-    // velocity solve over collision constraints in the same levels
-    for (int l = 0; l < L; ++l) {
-        for (int idx = lstart[l] + lane; idx < lstart[l + 1]; idx += 64) {
-            int r = cref[order[idx]];
-            if (r < 0) continue;
-            int2 p = w.pairs[w.live_global[r / MAXC].y];
+    // velocity solve over the collision items, in the same levels
+    for (int l = 0; l < Lmax; ++l) {
+        int i0 = (l < L) ? ls[l] : 0, i1 = (l < L) ? ls[l + 1] : 0;
+        for (int idx = i0 + lane; idx < i1; idx += LPS) {
+            int item = order[idx];
+            if (item < 0) continue;
+            int nc = w.ncontact[item];
+            if (nc == 0) continue;
+            int2 p = w.pairs[w.live_global[item].y];
             int bi1 = b0 + p.x, bi2 = b0 + p.y;
-            Body B1 = w.body[bi1];
-            Body B2 = w.body[bi2];
-            Contact c = w.contacts[r];
-            velocity_solve(c, B1, B2, h);
+            Body B1, B2;
+            load_vel_body(w.body[bi1], B1);
+            load_vel_body(w.body[bi2], B2);
+            const Contact* cs = w.contacts + (size_t)item * MAXC;
+            for (int c = 0; c < nc; ++c) {
+                Contact ct = cs[c];
+                velocity_solve(ct, B1, B2, h);
+            }
             if (!B1.fixed) { w.body[bi1].v = B1.v; w.body[bi1].w = B1.w; }
             if (!B2.fixed) { w.body[bi2].v = B2.v; w.body[bi2].w = B2.w; }
         }
         __syncthreads();
     }
-    (void)K;
-    if (err) atomicOr(&w.scene_err[s], err);
+    if (err && valid) atomicOr(&w.scene_err[s], err);
+}
+
+template <bool kExt>
+static void launch_solve(const DevWorld& w, const FrameParams& fp, hipStream_t st) {
+    int spw = 64 / w.solve_lanes;
+    dim3 grid((w.ns + spw - 1) / spw);
+    int it = (int)fp.iters, co = fp.collisions ? 1 : 0;
+    switch (w.solve_lanes) {
+    case 16: hipLaunchKernelGGL((k_solve<kExt, 16>), grid, dim3(64), 0, st, w, fp.h, it, co); break;
+    case 32: hipLaunchKernelGGL((k_solve<kExt, 32>), grid, dim3(64), 0, st, w, fp.h, it, co); break;
+    default: hipLaunchKernelGGL((k_solve<kExt, 64>), grid, dim3(64), 0, st, w, fp.h, it, co); break;
+    }
 }
 
 // Contacts of one body pair at the current transformed poses (debug / parity).
@@ -403,9 +570,9 @@ __global__ void k_pair_contacts(DevWorld w, int b1, int b2, double* out, int* co
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     HullView h1 = hull_view(w, b1), h2 = hull_view(w, b2);
     uint32_t err = 0;
-    int epa_fail = 0;
+    int flags = 0;
     ContactPt cts[MAXC];
-    int nc = pair_contacts(h1, h2, cts, err, epa_fail);
+    int nc = pair_contacts(h1, h2, cts, err, flags);
     for (int k = 0; k < nc; ++k) {
         double* o = out + 9 * k;
         o[0] = cts[k].p1.x; o[1] = cts[k].p1.y; o[2] = cts[k].p1.z;
@@ -451,21 +618,42 @@ hipError_t launch_frame(const DevWorld& w, const FrameParams& fp, hipStream_t st
     hipLaunchKernelGGL(k_islands, dim3(w.ns), dim3(64), 0, st, w, fp.dt);
     hipLaunchKernelGGL(k_scan_live, dim3(1), dim3(1024), 0, st, w);
     hipLaunchKernelGGL(k_scatter_live, dim3(w.ns), dim3(64), 0, st, w);
+    hipLaunchKernelGGL(k_levels, dim3(w.ns), dim3(64), 0, st, w, fp.collisions ? 1 : 0);
     t.end(st);
     for (unsigned sub = 0; sub < fp.substeps; ++sub) {
         t.begin(KC_INTEGRATE, st);
         hipLaunchKernelGGL(k_integrate, dim3(cdiv(w.nb, 128)), dim3(128), 0, st, w, fp.h);
         t.end(st);
         if (fp.collisions) {
-            t.begin(KC_TRANSFORM, st);
-            hipLaunchKernelGGL(k_transform, dim3(cdiv(w.n_xf, 256)), dim3(256), 0, st, w, 0);
-            t.end(st);
+            if (w.narrow_group == 1 || w.manifold_group == 1) {
+                t.begin(KC_TRANSFORM, st);
+                hipLaunchKernelGGL(k_transform, dim3(cdiv(w.n_xf, 256)), dim3(256), 0, st, w, 0);
+                t.end(st);
+            }
             t.begin(KC_NARROW, st);
-            hipLaunchKernelGGL(k_narrow, dim3(w.narrow_blocks), dim3(64), 0, st, w);
+            size_t lds = w.narrow_group > 1 ? (size_t)(64 / w.narrow_group) * w.stage_cap * sizeof(V3) : 0;
+            dim3 nb(w.narrow_blocks), t64(64);
+            switch (w.narrow_group) {
+            case 4: hipLaunchKernelGGL(k_gjk_epa<4>, nb, t64, lds, st, w); break;
+            case 8: hipLaunchKernelGGL(k_gjk_epa<8>, nb, t64, lds, st, w); break;
+            case 16: hipLaunchKernelGGL(k_gjk_epa<16>, nb, t64, lds, st, w); break;
+            default: hipLaunchKernelGGL(k_gjk_epa<1>, nb, t64, 0, st, w); break;
+            }
+            t.end(st);
+            t.begin(KC_MANIFOLD, st);
+            lds = w.manifold_group > 1 ? (size_t)(64 / w.manifold_group) * w.stage_cap * sizeof(V3) : 0;
+            dim3 mb(w.manifold_blocks);
+            switch (w.manifold_group) {
+            case 4: hipLaunchKernelGGL(k_manifold<4>, mb, t64, lds, st, w); break;
+            case 8: hipLaunchKernelGGL(k_manifold<8>, mb, t64, lds, st, w); break;
+            case 16: hipLaunchKernelGGL(k_manifold<16>, mb, t64, lds, st, w); break;
+            default: hipLaunchKernelGGL(k_manifold<1>, mb, t64, 0, st, w); break;
+            }
             t.end(st);
         }
         t.begin(KC_SOLVE, st);
-        hipLaunchKernelGGL(k_solve, dim3(w.ns), dim3(64), 0, st, w, fp.h, (int)fp.iters, fp.collisions ? 1 : 0);
+        if (w.n_ext > 0) launch_solve<true>(w, fp, st);
+        else launch_solve<false>(w, fp, st);
         t.end(st);
     }
     return hipGetLastError();
@@ -483,6 +671,19 @@ hipError_t launch_fp64_selftest(const double* a, const double* b, double* out, i
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_fp64_selftest, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, b, out, n);
     return hipGetLastError();
+}
+
+bool narrow_group_supported(const DevWorld& w, int g) {
+    if (g == 1) return true;
+    if (g != 4 && g != 8 && g != 16) return false;
+    return (size_t)(64 / g) * w.stage_cap * sizeof(V3) <= 64 * 1024;
+}
+
+int narrow_blocks_for(const DevWorld& w, int g, int64_t pcap) {
+    (void)w;
+    int64_t per_block = 64 / g;
+    int64_t blocks = (pcap + per_block - 1) / per_block;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(blocks, 256 * 16));
 }
 
 hipError_t launch_broad(const DevWorld& w, hipStream_t st) {

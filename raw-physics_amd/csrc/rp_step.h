@@ -86,32 +86,55 @@ constexpr int MAXC = 16;       // contacts kept per body pair
 // ---------------------------------------------------------------------------
 // Support mapping (support.cpp:5-39): first index wins on ties
 // ---------------------------------------------------------------------------
-// Vertices are loaded eight at a time so that their memory latencies
-// overlap; the comparisons still run in index order.
+// G = 1: one lane searches all vertices; they are loaded eight at a time so
+// that their memory latencies overlap, and compared in index order.
+// G > 1: the G lanes of an aligned group (one body pair per group) each scan
+// every G-th vertex, then reduce (larger dot wins, ties to the lower index),
+// which selects exactly the first maximum of the sequential scan; every lane
+// of the group returns the same index.
+template <int G = 1>
 RP_HD int support_index(const HullView& h, V3 d, uint32_t& err) {
     int sel = -1;
     double max_dot = -DBL_MAX;
-    int i = 0;
-    for (; i + 8 <= h.nv; i += 8) {
-        V3 v[8];
+    if constexpr (G == 1) {
+        int i = 0;
+        for (; i + 8 <= h.nv; i += 8) {
+            V3 v[8];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) v[k] = h.tv[i + k];
+            for (int k = 0; k < 8; ++k) v[k] = h.tv[i + k];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            double dd = dot(v[k], d);
-            if (dd > max_dot) { sel = i + k; max_dot = dd; }
+            for (int k = 0; k < 8; ++k) {
+                double dd = dot(v[k], d);
+                if (dd > max_dot) { sel = i + k; max_dot = dd; }
+            }
         }
-    }
-    for (; i < h.nv; ++i) {
-        double dd = dot(h.tv[i], d);
-        if (dd > max_dot) { sel = i; max_dot = dd; }
+        for (; i < h.nv; ++i) {
+            double dd = dot(h.tv[i], d);
+            if (dd > max_dot) { sel = i; max_dot = dd; }
+        }
+    } else {
+        int gl = (int)(__lane_id() & (G - 1));
+        for (int i = gl; i < h.nv; i += G) {
+            double dd = dot(h.tv[i], d);
+            if (dd > max_dot) { sel = i; max_dot = dd; }
+        }
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1) {
+            double ob = __shfl_xor(max_dot, off, G);
+            int os = __shfl_xor(sel, off, G);
+            if (os >= 0 && (sel < 0 || ob > max_dot || (ob == max_dot && os < sel))) {
+                max_dot = ob;
+                sel = os;
+            }
+        }
     }
     if (sel < 0) { err |= ERR_SUPPORT_NAN; sel = 0; }
     return sel;
 }
+template <int G = 1>
 RP_HD V3 support_md(const HullView& a, const HullView& b, V3 d, uint32_t& err) {
-    V3 s1 = a.tv[support_index(a, d, err)];
-    V3 s2 = b.tv[support_index(b, scl(-1.0, d), err)];
+    V3 s1 = a.tv[support_index<G>(a, d, err)];
+    V3 s2 = b.tv[support_index<G>(b, scl(-1.0, d), err)];
     return sub(s1, s2);
 }
 
@@ -209,14 +232,49 @@ RP_HD bool do_simplex_4(Simplex& s, V3& dir) {
     return false;
 }
 
-// gjk_collides (gjk.cpp:350-379)
-RP_HD bool gjk(const HullView& h1, const HullView& h2, Simplex& s, uint32_t& err) {
-    s.a = support_md(h1, h2, v3(0.0, 0.0, 1.0), err);
+RP_HD bool same_bits(V3 a, V3 b) {
+    return __double_as_longlong(a.x) == __double_as_longlong(b.x) &&
+           __double_as_longlong(a.y) == __double_as_longlong(b.y) &&
+           __double_as_longlong(a.z) == __double_as_longlong(b.z);
+}
+RP_HD bool same_state(const Simplex& s, V3 dir, const Simplex& t, V3 tdir) {
+    return s.num == t.num && same_bits(dir, tdir) && same_bits(s.a, t.a) && same_bits(s.b, t.b) &&
+           same_bits(s.c, t.c) && same_bits(s.d, t.d);
+}
+
+// gjk_collides (gjk.cpp:350-379).
+// Early exit on a repeated state, with the same result as the full loop: the
+// loop state (simplex, its count and the search direction) determines the
+// rest of the loop, and it takes values in a finite set (support points are
+// differences of hull vertices, the direction a function of the simplex). A
+// run that neither separates nor encloses the origin therefore cycles until
+// its 100th iteration and returns false. Brent's method finds the cycle: the
+// state is saved at iterations 1, 2, 4, 8, ... and compared bit for bit with
+// every later state; a match means the iterations since the save returned
+// nothing and would repeat until the limit (their error bits are already
+// set), so false is returned at once. `cycled` is set when this happens.
+template <int G = 1>
+RP_HD bool gjk(const HullView& h1, const HullView& h2, Simplex& s, uint32_t& err, bool& cycled) {
+    s.a = support_md<G>(h1, h2, v3(0.0, 0.0, 1.0), err);
     s.b = s.c = s.d = v3(0.0, 0.0, 0.0);
     s.num = 1;
     V3 dir = scl(-1.0, s.a);
+    Simplex saved = s;
+    V3 saved_dir = dir;
+    int power = 1, lam = 0;
     for (int i = 0; i < 100; ++i) {
-        V3 p = support_md(h1, h2, dir, err);
+        if (lam == power) {
+            saved = s;
+            saved_dir = dir;
+            power <<= 1;
+            lam = 0;
+        }
+        ++lam;
+        if (i > 0 && same_state(s, dir, saved, saved_dir) && lam > 1) {
+            cycled = true;
+            return false;
+        }
+        V3 p = support_md<G>(h1, h2, dir, err);
         if (dot(p, dir) < 0.0) return false;
         // add_to_simplex (gjk.cpp:7-30); num == 4 is an assert in the reference
         switch (s.num) {
@@ -268,6 +326,7 @@ RP_HD bool epa_face(I3 f, const V3* poly, int np, V3& n_out, double& d_out) {
 }
 
 // epa (epa.cpp:118-237): 0 = no convergence, 1 = converged, -1 = error
+template <int G = 1>
 RP_HD int epa(const HullView& h1, const HullView& h2, const Simplex& s, V3& n_out, double& pen_out, uint32_t& err) {
     V3 poly[EPA_MAXV];
     I3 faces[EPA_MAXF];
@@ -287,7 +346,7 @@ RP_HD int epa(const HullView& h1, const HullView& h2, const Simplex& s, V3& n_ou
         if (d < min_d) { min_d = d; min_n = n; }
     }
     for (int it = 0; it < 100; ++it) {
-        V3 sp = support_md(h1, h2, min_n, err);
+        V3 sp = support_md<G>(h1, h2, min_n, err);
         double d = dot(min_n, sp);
         if (fabs(d - min_d) < 0.0001) {
             n_out = min_n;
@@ -435,11 +494,12 @@ struct ContactPt { V3 p1, p2, n; };
 
 // convex_convex_contact_manifold (clipping.cpp:249-341). Returns the number
 // of contacts written to `out` (at most MAXC).
+template <int G = 1>
 RP_HD int manifold(const HullView& h1, const HullView& h2, V3 normal, ContactPt* out, uint32_t& err) {
     const double EPSILON = 0.0001;
     V3 inormal = inv3(normal);
-    int s1 = support_index(h1, normal, err);
-    int s2 = support_index(h2, inormal, err);
+    int s1 = support_index<G>(h1, normal, err);
+    int s2 = support_index<G>(h2, inormal, err);
     int f1 = best_face(s1, h1, normal, err);
     int f2 = best_face(s2, h2, inormal, err);
     // get_edge_with_most_fitting_normal (clipping.cpp:154-200)
@@ -531,13 +591,20 @@ RP_HD int manifold(const HullView& h1, const HullView& h2, V3 normal, ContactPt*
 // collider_get_contacts for two hulls (collider.cpp:523-558): GJK, EPA, manifold.
 // `epa_fail` counts EPA runs that did not converge (the reference prints and
 // drops the pair's contacts, epa.cpp:233-236, collider.cpp:549-551).
-RP_HD int pair_contacts(const HullView& h1, const HullView& h2, ContactPt* out, uint32_t& err, int& epa_fail) {
+// `flags` gets PAIR_EPA_FAIL and PAIR_GJK_CYCLE (a cycle exit of gjk).
+enum { PAIR_EPA_FAIL = 1, PAIR_GJK_CYCLE = 2 };
+template <int G = 1>
+RP_HD int pair_contacts(const HullView& h1, const HullView& h2, ContactPt* out, uint32_t& err, int& flags) {
     Simplex s;
-    if (!gjk(h1, h2, s, err)) return 0;
+    bool cycled = false;
+    if (!gjk<G>(h1, h2, s, err, cycled)) {
+        if (cycled) flags |= PAIR_GJK_CYCLE;
+        return 0;
+    }
     V3 n; double pen;
-    int r = epa(h1, h2, s, n, pen, err);
-    if (r != 1) { epa_fail = 1; return 0; }
-    return manifold(h1, h2, n, out, err);
+    int r = epa<G>(h1, h2, s, n, pen, err);
+    if (r != 1) { flags |= PAIR_EPA_FAIL; return 0; }
+    return manifold<G>(h1, h2, n, out, err);
 }
 
 // ---------------------------------------------------------------------------

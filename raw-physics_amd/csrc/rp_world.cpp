@@ -253,6 +253,7 @@ struct DeviceImage {
     DevWorld d;
     std::vector<void*> allocs;
     std::vector<int32_t> body_tv_off;  // host copy for inspection
+    int64_t pair_capacity = 0;
     bool ok = false;
 
     template <class T>
@@ -290,6 +291,7 @@ static bool build_image(const HostWorld& hw, DeviceImage& im) {
     memset(&d, 0, sizeof(d));
     d.ns = (int)hw.scene_body_off.size() - 1;
     d.nb = (int)hw.body.size();
+    d.n_ext = (int)hw.ext.size();
     // topology dedupe: bodies of the same shape share index lists
     std::map<std::vector<int32_t>, int32_t> topo_ids;
     std::vector<int32_t> topo_base, topo_pool;
@@ -339,7 +341,7 @@ static bool build_image(const HostWorld& hw, DeviceImage& im) {
         pair_base[s] = (int32_t)pcap;
         con_base[s] = (int32_t)ccap;
         pcap += n * (n - 1) / 2;
-        ccap += next + (n * (n - 1) / 2) * MAXC;
+        ccap += next + n * (n - 1) / 2;
         if (pcap > INT32_MAX / MAXC || ccap > INT32_MAX - d.ns - 1) {
             set_msg("world too large for 32-bit pair/constraint indices");
             return false;
@@ -347,13 +349,23 @@ static bool build_image(const HostWorld& hw, DeviceImage& im) {
     }
     pair_base[d.ns] = (int32_t)pcap;
     con_base[d.ns] = (int32_t)ccap;
-    d.narrow_blocks = (int)std::max<int64_t>(1, std::min<int64_t>((pcap + 63) / 64, 8192));
+    int max_nv = 1;
+    for (const HostHull& h : hw.hulls) max_nv = std::max(max_nv, (int)h.v.size());
+    d.stage_cap = 2 * max_nv;
+    im.pair_capacity = pcap;
+    d.narrow_group = 1;
+    for (int g : {8, 16, 4})
+        if (narrow_group_supported(d, g)) { d.narrow_group = g; break; }
+    d.narrow_blocks = narrow_blocks_for(d, d.narrow_group, pcap);
+    d.manifold_group = 1;
+    d.manifold_blocks = narrow_blocks_for(d, d.manifold_group, pcap);
+    d.solve_lanes = 64;
 
     bool ok = true;
     ok &= im.upload(d.scene_body_off, hw.scene_body_off);
     ok &= im.upload(d.scene_pair_base, pair_base);
     ok &= im.upload(d.scene_ext_off, hw.scene_ext_off);
-    ok &= im.upload(d.scene_con_base, con_base);
+    ok &= im.upload(d.scene_item_base, con_base);
     ok &= im.upload_mut(d.body, hw.body);
     ok &= im.alloc(d.model, d.nb);
     ok &= im.upload(d.body_hull, hw.body_hull);
@@ -378,13 +390,17 @@ static bool build_image(const HostWorld& hw, DeviceImage& im) {
     ok &= im.alloc(d.live_prefix, d.ns + 1);
     ok &= im.alloc(d.live_global, (size_t)pcap);
     ok &= im.alloc(d.ncontact, (size_t)pcap);
+    ok &= im.alloc(d.pair_normal, (size_t)pcap);
+    ok &= im.alloc(d.mlist, (size_t)pcap);
+    ok &= im.alloc(d.counters, CNT_COUNT);
     ok &= im.alloc(d.contacts, (size_t)pcap * MAXC);
     ok &= im.upload(d.ext, hw.ext);
     ok &= im.alloc(d.ext_lambda, 3 * hw.ext.size());
-    ok &= im.alloc(d.con_ref, (size_t)ccap);
-    ok &= im.alloc(d.con_level, (size_t)ccap);
-    ok &= im.alloc(d.con_order, (size_t)ccap);
-    ok &= im.alloc(d.con_lstart, (size_t)ccap + d.ns);
+    ok &= im.alloc(d.item_level, (size_t)ccap);
+    ok &= im.alloc(d.item_order, (size_t)ccap);
+    ok &= im.alloc(d.item_lstart, (size_t)ccap + d.ns);
+    ok &= im.alloc(d.item_cursor, (size_t)ccap + d.ns);
+    ok &= im.alloc(d.scene_nlevels, d.ns);
     ok &= im.alloc(d.body_last, d.nb);
     ok &= im.alloc(d.uf_parent, d.nb);
     ok &= im.alloc(d.uf_flag, d.nb);
@@ -499,7 +515,43 @@ struct rp_world {
     KernelTimer timer;              // profiling mode
     double kc_ms[KC_COUNT] = {0};
     uint64_t kc_launches[KC_COUNT] = {0};
+    // one frame's launches captured as a HIP graph and replayed per frame
+    bool use_graphs = true;
+    hipGraphExec_t frame_exec = nullptr;
+    FrameParams graph_fp;
+    uint64_t sched_gen = 1, graph_gen = 0;  // schedule changes invalidate the captured graph
 };
+
+static bool same_params(const FrameParams& a, const FrameParams& b) {
+    return a.dt == b.dt && a.h == b.h && a.gravity == b.gravity && a.substeps == b.substeps && a.iters == b.iters &&
+           a.collisions == b.collisions && a.forces_given == b.forces_given;
+}
+
+// The graph of one frame for these parameters and the current schedules,
+// captured once and reused while they do not change.
+static hipError_t frame_graph(rp_world* w, const FrameParams& fp) {
+    const DevWorld& d = w->im.d;
+    if (w->frame_exec && same_params(w->graph_fp, fp) && w->graph_gen == w->sched_gen) return hipSuccess;
+    if (w->frame_exec) {
+        (void)hipGraphExecDestroy(w->frame_exec);
+        w->frame_exec = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    hipError_t le = launch_frame(d, fp, w->stream, nullptr);
+    e = hipStreamEndCapture(w->stream, &g);
+    if (le != hipSuccess) e = le;
+    if (e == hipSuccess) e = hipGraphInstantiate(&w->frame_exec, g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        w->frame_exec = nullptr;
+        return e;
+    }
+    w->graph_fp = fp;
+    w->graph_gen = w->sched_gen;
+    return hipSuccess;
+}
 
 static bool world_init(rp_world* w) {
     if (hipStreamCreate(&w->stream) != hipSuccess) {
@@ -512,6 +564,7 @@ static bool world_init(rp_world* w) {
 static void world_free(rp_world* w) {
     if (!w) return;
     if (w->stream) (void)hipStreamSynchronize(w->stream);
+    if (w->frame_exec) (void)hipGraphExecDestroy(w->frame_exec);
     for (hipEvent_t e : w->timer.pool) (void)hipEventDestroy(e);
     w->im.release();
     if (w->stream) (void)hipStreamDestroy(w->stream);
@@ -519,6 +572,15 @@ static void world_free(rp_world* w) {
 }
 
 static uint32_t world_run(rp_world* w, const FrameParams& fp, uint32_t frames, bool sync) {
+    if (w->use_graphs && !w->timer.on && frames > 0) {
+        hipError_t e = frame_graph(w, fp);
+        for (uint32_t f = 0; e == hipSuccess && f < frames; ++f) e = hipGraphLaunch(w->frame_exec, w->stream);
+        if (e != hipSuccess) {
+            set_msg(std::string("frame graph: ") + hipGetErrorString(e));
+            return RP_ERR_RUNTIME;
+        }
+        frames = 0;
+    }
     for (uint32_t f = 0; f < frames; ++f) {
         w->timer.used = 0;
         hipError_t e = launch_frame(w->im.d, fp, w->stream, &w->timer);
@@ -654,12 +716,14 @@ uint32_t rp_world_error(rp_world* w, uint32_t scene) {
     return e;
 }
 
-uint64_t rp_world_epa_failures(rp_world* w) {
+static uint64_t read_stat(rp_world* w, int k) {
     unsigned long long v = 0;
     if (hipStreamSynchronize(w->stream) != hipSuccess) return 0;
-    if (hipMemcpy(&v, w->im.d.stats + STAT_EPA_FAIL, 8, hipMemcpyDeviceToHost) != hipSuccess) return 0;
+    if (hipMemcpy(&v, w->im.d.stats + k, 8, hipMemcpyDeviceToHost) != hipSuccess) return 0;
     return v;
 }
+uint64_t rp_world_epa_failures(rp_world* w) { return read_stat(w, STAT_EPA_FAIL); }
+uint64_t rp_world_gjk_cycle_exits(rp_world* w) { return read_stat(w, STAT_GJK_CYCLE); }
 
 void rp_world_body_info(rp_world* w, double* inv_mass, double* inertia, double* inv_inertia, double* radius) {
     for (size_t i = 0; i < w->hw.body.size(); ++i) {
@@ -746,6 +810,43 @@ uint32_t rp_fp64_selftest(const double* a, const double* b, double* out, uint64_
     if (db) (void)hipFree(db);
     if (dout) (void)hipFree(dout);
     return rc;
+}
+
+int32_t rp_world_set_narrow_group(rp_world* w, int32_t lanes_per_pair) {
+    if (!narrow_group_supported(w->im.d, lanes_per_pair)) return -1;
+    (void)hipStreamSynchronize(w->stream);
+    w->im.d.narrow_group = lanes_per_pair;
+    w->im.d.narrow_blocks = narrow_blocks_for(w->im.d, lanes_per_pair, w->im.pair_capacity);
+    ++w->sched_gen;
+    return 0;
+}
+
+int32_t rp_world_narrow_group(const rp_world* w) { return w->im.d.narrow_group; }
+
+int32_t rp_world_set_manifold_group(rp_world* w, int32_t lanes_per_pair) {
+    if (!narrow_group_supported(w->im.d, lanes_per_pair)) return -1;
+    (void)hipStreamSynchronize(w->stream);
+    w->im.d.manifold_group = lanes_per_pair;
+    w->im.d.manifold_blocks = narrow_blocks_for(w->im.d, lanes_per_pair, w->im.pair_capacity);
+    ++w->sched_gen;
+    return 0;
+}
+
+int32_t rp_world_manifold_group(const rp_world* w) { return w->im.d.manifold_group; }
+
+int32_t rp_world_set_solve_lanes(rp_world* w, int32_t lanes_per_scene) {
+    if (lanes_per_scene != 16 && lanes_per_scene != 32 && lanes_per_scene != 64) return -1;
+    (void)hipStreamSynchronize(w->stream);
+    w->im.d.solve_lanes = lanes_per_scene;
+    ++w->sched_gen;
+    return 0;
+}
+
+int32_t rp_world_solve_lanes(const rp_world* w) { return w->im.d.solve_lanes; }
+
+void rp_world_set_graphs(rp_world* w, int32_t on) {
+    (void)hipStreamSynchronize(w->stream);
+    w->use_graphs = on != 0;
 }
 
 void rp_world_set_profiling(rp_world* w, int32_t on) {

@@ -459,3 +459,87 @@ def chains(num_scenes: int, length: int = 6, first_seed: int = 0, **kw) -> World
     for s in range(num_scenes):
         chain_scene(w, first_seed + s, length=length, **kw)
     return w
+
+
+def uneven_scene(w: World, seed: int, n_min: int = 4, n_max: int = 256, chain_fraction: float = 0.3,
+                 max_bodies: int | None = None) -> World:
+    """One batched-uneven scene: N = round(exp(U(ln n_min, ln n_max))) dynamic
+    bodies (capped by `max_bodies`), each a cube (8 vertices), a 42-vertex or
+    a 162-vertex icosphere with size r ~ U(0.5, 1.0) and mass = hull volume,
+    on a jittered lattice above the floor (no initial overlap); per body
+    mu_s = mu_d ~ U(0.2, 0.8) (one draw, so dynamic never exceeds static,
+    entity.cpp:58-60) and restitution ~ U(0, 0.5); v ~ N(0, 0.5)^3. With
+    probability `chain_fraction` the scene also carries a chain of U{2..12}
+    box links hanging from a static support beside the pile, joined
+    alternately by limited hinge and spherical joints (chain_scene)."""
+    rng = Rng(0x5EED0000 + seed)
+    n = int(round(math.exp(rng.uniform(math.log(n_min), math.log(n_max)))))
+    if max_bodies is not None:
+        n = min(n, max_bodies)
+    has_chain = rng.uniform() < chain_fraction
+    chain_len = 2 + int(rng.uniform() * 11) if has_chain else 0
+    _floor(w)
+    shapes = [w.shape(cube_corners()), w.shape(icosphere_corners(1)), w.shape(icosphere_corners(2))]
+    unit_volume = {sh: hull_volume(w.shapes[sh], (1.0, 1.0, 1.0)) for sh in shapes}
+    side = max(1, int(math.ceil(math.sqrt(min(n, 49)))))
+    for k in range(n):
+        kind = min(2, int(rng.uniform() * 3))
+        shape = shapes[kind]
+        r = rng.uniform(0.5, 1.0)
+        gx, gz = k % side, (k // side) % side
+        layer = k // (side * side)
+        x = (gx - (side - 1) / 2.0) * GRID_SPACING + rng.uniform(-0.2, 0.2)
+        z = (gz - (side - 1) / 2.0) * GRID_SPACING + rng.uniform(-0.2, 0.2)
+        bound = math.sqrt(3.0) * r if kind == 0 else r
+        y = -1.0 + bound + 1.0 + GRID_SPACING * layer + rng.uniform(0.0, 0.3)
+        mu = rng.uniform(0.2, 0.8)
+        e = rng.uniform(0.0, 0.5)
+        q = rng.quaternion()
+        v = (rng.normal(0.5), rng.normal(0.5), rng.normal(0.5))
+        w.add_body(shape, (r, r, r), (x, y, z), q, mass=unit_volume[shape] * r * r * r, material=(mu, mu, e),
+                   linear_velocity=v)
+    if has_chain:
+        # the chain hangs beside the pile, clear of the lattice
+        base = len(w.body_shape) - w.scene_body_offset[-1]
+        link = (0.2, 0.5, 0.1)
+        ox = (side / 2.0 + 1.5) * GRID_SPACING
+        sup_pos = (ox, 1.0 + 2.0 * link[1] * chain_len + 0.5, 0.0)
+        ident = quaternion_new((1.0, 0.0, 0.0), 0.0)
+        cube = shapes[0]
+        prev = w.add_body(cube, (0.2, 0.1, 0.1), sup_pos, ident, fixed=True, material=(0.5, 0.5, 0.0))
+        prev_pos, prev_q, prev_half = sup_pos, ident, 0.1
+        for k in range(chain_len):
+            jit = lambda: rng.uniform(-0.05, 0.05)  # noqa: E731
+            r1 = (jit(), -prev_half + jit(), jit())
+            r2 = (jit(), link[1] + jit(), jit())
+            q = quaternion_new((0.0, 1.0, 0.0), rng.uniform(-10.0, 10.0))
+            p1 = tuple(a + b for a, b in zip(prev_pos, _qrot(prev_q, r1)))
+            p2 = tuple(a + b for a, b in zip(prev_pos, _qrot(q, r2)))
+            pos = tuple(x + (a - b) for x, a, b in zip(prev_pos, p1, p2))
+            v = (rng.normal(1.0), rng.normal(1.0), rng.normal(1.0))
+            body = w.add_body(cube, link, pos, q, mass=1.0, material=(0.6, 0.6, 0.0), linear_velocity=v)
+            c = ConstraintDesc()
+            c.body1, c.body2 = prev, body
+            c.r1_lc[:] = r1
+            c.r2_lc[:] = r2
+            if k % 2 == 0:
+                c.type = HINGE
+                c.axis[:] = (POS_Z, POS_Z, POS_Y, POS_Y)
+                c.limited = 1
+                c.limits[:] = (-PI_F / 2.0, PI_F / 2.0, 0.0, 0.0)
+            else:
+                c.type = SPHERICAL
+                c.axis[:] = (POS_X, POS_X, POS_Y, POS_Y)
+                c.limits[:] = (-PI_F / 4.0, PI_F / 4.0, -PI_F / 8.0, PI_F / 8.0)
+            w.add_constraint(c)
+            prev, prev_pos, prev_q, prev_half = body, pos, q, link[1]
+        _ = base
+    w.end_scene()
+    return w
+
+
+def batched_uneven(num_scenes: int, first_seed: int = 0, **kw) -> World:
+    w = World(name=f"batched_uneven_{num_scenes}")
+    for s in range(num_scenes):
+        uneven_scene(w, first_seed + s, **kw)
+    return w
