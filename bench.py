@@ -145,6 +145,9 @@ def main():
     ap.add_argument("--manifold-lanes", type=int, default=0, help="fixed manifold schedule (lanes per pair); skips the autotuner")
     ap.add_argument("--solve-lanes", type=int, default=0, help="fixed solve schedule (lanes per scene); skips the autotuner")
     ap.add_argument("--eager", action="store_true", help="launch each kernel eagerly instead of replaying frame graphs")
+    ap.add_argument("--save-state", metavar="NPZ", help="after the timed frames, save the batch's body state")
+    ap.add_argument("--load-state", metavar="NPZ",
+                    help="start the timed frames from a saved body state (a window deep in the run, for profiles)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run one device-to-device copy of MIB MiB (a kernel of known traffic that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
@@ -168,6 +171,10 @@ def main():
     t0 = time.perf_counter()
     gw = pkg.World(world)
     log(f"uploaded world in {time.perf_counter() - t0:.1f} s")
+    if args.load_state:
+        with np.load(args.load_state, allow_pickle=False) as z:
+            gw.set_state({k: z[k] for k in z.files})
+        log(f"start state loaded from {args.load_state}")
     init = gw.state()
 
     # schedule: the package's autotuner times each narrow-phase variant on a
@@ -201,6 +208,9 @@ def main():
     elapsed = time.perf_counter() - t0
     dyn = world.dynamic_bodies()
     value = dyn * substeps * args.steps / elapsed
+    if args.save_state:
+        np.savez(args.save_state, **gw.state())
+        log(f"state after the timed frames saved to {args.save_state}")
     errs = gw.errors()
     narrow_group, manifold_group, solve_lanes = gw.narrow_group, gw.manifold_group, gw.solve_lanes
     log(f"batched: {args.steps} frames in {elapsed:.3f} s -> {value:.4g} body-substeps/s; "

@@ -183,6 +183,12 @@ typedef struct {
     const double* body_material;          /* [3nb] static friction, dynamic friction, restitution */
     const uint32_t* scene_constraint_offset; /* [num_scenes+1] or NULL */
     const rp_constraint_desc* constraints;   /* scene-grouped external constraints or NULL */
+    /* Scenes with more bodies than this (0: 4096) run the large-scene path:
+     * uniform-grid broad phase (same pair list as the all-pairs test),
+     * parallel islands, and a graph-coloured solve instead of the reference's
+     * sequential constraint order (results then differ from the reference
+     * within a tolerance, and do not depend on thread timing). */
+    uint32_t large_scene_bodies;
 } rp_world_desc;
 
 /* Per-body state as flat arrays; NULL members are skipped. */
@@ -215,6 +221,7 @@ enum {
     RP_ERR_CLIP_CAPACITY = 1u << 9,   /* clipped polygon exceeded its fixed capacity */
     RP_ERR_CONTACT_CAPACITY = 1u << 10, /* more than 16 contacts for one body pair */
     RP_ERR_SOLVER_CAPACITY = 1u << 11,  /* constraint list exceeded its capacity */
+    RP_ERR_PAIR_CAPACITY = 1u << 12,    /* broad-phase pairs of a large scene exceeded its capacity */
     RP_ERR_RUNTIME = 1u << 31         /* HIP runtime failure (see rp_last_error_string) */
 };
 

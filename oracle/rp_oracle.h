@@ -70,6 +70,7 @@ typedef struct {
     const double* body_material;          /* [3nb] static friction, dynamic friction, restitution */
     const uint32_t* scene_constraint_offset; /* [num_scenes+1] or NULL */
     const orc_constraint_desc* constraints;  /* external constraints, scene-grouped */
+    uint32_t large_scene_bodies;             /* the device path's large-scene threshold (unused here) */
 } orc_world_desc;
 
 /* Per-body state, read and written as flat arrays (any pointer may be NULL). */

@@ -16,8 +16,10 @@ from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
 LIB = HERE / "librawphys.so"
-SOURCES = [HERE / "csrc" / "rp_kernels.hip", HERE / "csrc" / "rp_world.cpp", HERE / "csrc" / "rp_refsyms.cpp"]
-HEADERS = [HERE / "csrc" / "rp_kernels.h", HERE / "csrc" / "rp_step.h", HERE / "csrc" / "rp_math.h",
+SOURCES = [HERE / "csrc" / "rp_kernels.hip", HERE / "csrc" / "rp_large.hip", HERE / "csrc" / "rp_world.cpp",
+           HERE / "csrc" / "rp_refsyms.cpp"]
+HEADERS = [HERE / "csrc" / "rp_kernels.h", HERE / "csrc" / "rp_device.h", HERE / "csrc" / "rp_step.h",
+           HERE / "csrc" / "rp_math.h",
            HERE.parent / "include" / "rawphys.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -1,0 +1,92 @@
+// rp_device.h — device helpers shared by the small-scene and large-scene kernels.
+#pragma once
+#include "rp_kernels.h"
+
+namespace rp {
+
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+// Ordered wave compaction: returns this lane's output slot among lanes with
+// pred set (in lane order) and the wave total in *total.
+__device__ __forceinline__ int wave_compact(bool pred, int* total) {
+    unsigned long long m = __ballot(pred);
+    *total = __popcll(m);
+    unsigned long long below = (lane_id() == 0) ? 0ull : (m & ((~0ull) >> (64 - lane_id())));
+    return __popcll(below);
+}
+
+// Loads of the body fields each solve reads (the rest of Body is unused).
+__device__ __forceinline__ void load_pos_body(const Body& g, Body& b) {
+    b.x = g.x; b.q = g.q; b.px = g.px; b.pq = g.pq;
+    b.inv_mass = g.inv_mass; b.Iinv = g.Iinv; b.mu_s = g.mu_s; b.fixed = g.fixed;
+}
+__device__ __forceinline__ void load_vel_body(const Body& g, Body& b) {
+    b.q = g.q; b.v = g.v; b.w = g.w; b.pv = g.pv; b.pw = g.pw;
+    b.inv_mass = g.inv_mass; b.Iinv = g.Iinv; b.mu_d = g.mu_d; b.e = g.e; b.fixed = g.fixed;
+}
+
+// The broad-phase test of two bodies (broad.cpp:17-21), body i first.
+__device__ __forceinline__ bool broad_hit(const Body& e1, const Body& e2) {
+    double dd = len(sub(e1.x, e2.x));
+    double maxd = e1.radius + e2.radius + 0.1;
+    return dd <= maxd;
+}
+
+// One solve item: an external constraint (item < 0, index ~item) or the
+// contacts of live pair `item` in order (pbd.cpp:381-406, 615-620). b0 is
+// the scene's first body.
+template <bool kExt>
+__device__ __forceinline__ void solve_item_pos(const DevWorld& w, int item, int b0, double h, uint32_t& err) {
+    int bi1, bi2;
+    if (item < 0) { bi1 = w.ext[~item].b1; bi2 = w.ext[~item].b2; }
+    else { int2 p = w.pairs[w.live_global[item].y]; bi1 = b0 + p.x; bi2 = b0 + p.y; }
+    Body B1, B2;
+    load_pos_body(w.body[bi1], B1);
+    load_pos_body(w.body[bi2], B2);
+    if (item < 0) {
+        if constexpr (kExt) solve_external(w.ext[~item], w.ext_lambda + 3 * (~item), B1, B2, h, err);
+    } else {
+        int nc = w.ncontact[item];
+        Contact* cs = w.contacts + (size_t)item * MAXC;
+        for (int c = 0; c < nc; ++c) {
+            Contact ct = cs[c];
+            solve_collision(ct, B1, B2, h, err);
+            cs[c].lambda_n = ct.lambda_n;
+            cs[c].lambda_t = ct.lambda_t;
+        }
+    }
+    if (!B1.fixed) { w.body[bi1].x = B1.x; w.body[bi1].q = B1.q; }
+    if (!B2.fixed) { w.body[bi2].x = B2.x; w.body[bi2].q = B2.q; }
+}
+
+// Velocity solve of the contacts of one item (pbd.cpp:646-711); external
+// constraints take no part.
+__device__ __forceinline__ void solve_item_vel(const DevWorld& w, int item, int b0, double h) {
+    if (item < 0) return;
+    int nc = w.ncontact[item];
+    if (nc == 0) return;
+    int2 p = w.pairs[w.live_global[item].y];
+    int bi1 = b0 + p.x, bi2 = b0 + p.y;
+    Body B1, B2;
+    load_vel_body(w.body[bi1], B1);
+    load_vel_body(w.body[bi2], B2);
+    const Contact* cs = w.contacts + (size_t)item * MAXC;
+    for (int c = 0; c < nc; ++c) {
+        Contact ct = cs[c];
+        velocity_solve(ct, B1, B2, h);
+    }
+    if (!B1.fixed) { w.body[bi1].v = B1.v; w.body[bi1].w = B1.w; }
+    if (!B2.fixed) { w.body[bi2].v = B2.v; w.body[bi2].w = B2.w; }
+}
+
+// Velocity update of one body (pbd.cpp:623-643).
+__device__ __forceinline__ void velocity_update_body(Body& g, double h) {
+    if (g.fixed || !g.active) return;
+    Body e;
+    e.x = g.x; e.px = g.px; e.q = g.q; e.pq = g.pq; e.v = g.v; e.w = g.w;
+    e.fixed = 0; e.active = 1;
+    velocity_update(e, h);
+    g.pv = e.pv; g.pw = e.pw; g.v = e.v; g.w = e.w;
+}
+
+}  // namespace rp

@@ -60,9 +60,38 @@ struct DevWorld {
     int32_t* scene_nlevels;         // [ns]
     int32_t* body_last;             // [nb]          last level per body (large scenes)
     int32_t *uf_parent, *uf_flag;   // [nb]
+    const int32_t* scene_big;       // [ns] 1: large scene (grid broad phase, coloured solve)
     // diagnostics
     uint32_t* scene_err;            // [ns]
     unsigned long long* stats;      // [STAT_COUNT]
+};
+
+// A large scene: grid broad phase, parallel islands and a graph-coloured
+// solve instead of the per-scene wave kernels. Device buffers are owned by
+// the world.
+constexpr int BIG_COLOURS = 24;             // parallel colours; items beyond them go to one serial class
+constexpr int BIG_SERIAL = BIG_COLOURS;     // class solved by one lane in item order
+constexpr int BIG_CLASSES = BIG_COLOURS + 1;
+constexpr int BIG_COLOUR_ROUNDS = 32;       // Jones-Plassmann rounds per frame
+struct BigScene {
+    int s, b0, n;                // scene, first body, bodies
+    int pair_base, pair_cap;     // slice of the pair arrays
+    int item_base, item_cap;     // slice of the item arrays
+    int T;                       // hash-table buckets (power of two)
+    int n_large;                 // bodies tested against all others (radius above the cell bound)
+    double cell;                 // grid cell edge
+    int32_t* large;              // [n_large] scene-local indices, ascending
+    int32_t* is_large;           // [n]
+    int32_t* cell_start;         // [T + 1]
+    int32_t* cell_cursor;        // [T]
+    int32_t* sorted;             // [n] small bodies grouped by bucket
+    int32_t* pair_off;           // [n + 1] per-body pair counts, then offsets
+    int32_t* tmp;                // [max(pair_cap, item_cap) + 1] scan buffer
+    int32_t* bi_start;           // [n + 1] body -> items CSR
+    int32_t* bi_cursor;          // [n]
+    int32_t* bi_items;           // [2 item_cap]
+    int32_t* remaining;          // [1] uncoloured items
+    int32_t* scan_blocks;        // [scan block sums]
 };
 
 struct FrameParams {
@@ -102,10 +131,19 @@ struct KernelTimer {
     }
 };
 
-hipError_t launch_frame(const DevWorld& w, const FrameParams& fp, hipStream_t st, KernelTimer* kt = nullptr);
+hipError_t launch_frame(const DevWorld& w, const FrameParams& fp, hipStream_t st, KernelTimer* kt = nullptr,
+                        const BigScene* bigs = nullptr, int nbig = 0);
+// Large-scene stages (rp_large.hip), in frame order.
+void big_broad(const DevWorld& w, const BigScene& b, hipStream_t st);
+void big_islands(const DevWorld& w, const BigScene& b, double dt, hipStream_t st);
+void big_scatter(const DevWorld& w, const BigScene& b, hipStream_t st);
+void big_colour(const DevWorld& w, const BigScene& b, bool collisions, hipStream_t st);
+void big_solve(const DevWorld& w, const BigScene& b, const FrameParams& fp, hipStream_t st);
+// Blocks of the device-wide scan for n elements (size of BigScene::scan_blocks).
+int scan_block_count(int64_t n);
 hipError_t launch_pair_contacts(const DevWorld& w, int b1, int b2, double* out, int* count, uint32_t* err,
                                 hipStream_t st);
-hipError_t launch_broad(const DevWorld& w, hipStream_t st);
+hipError_t launch_broad(const DevWorld& w, hipStream_t st, const BigScene* bigs = nullptr, int nbig = 0);
 // Lanes per pair the narrow phase can use for this world (LDS permitting).
 bool narrow_group_supported(const DevWorld& w, int g);
 int narrow_blocks_for(const DevWorld& w, int g, int64_t pair_capacity);

@@ -22,20 +22,9 @@
 // sequence of updates as in the sequential loop and results are bit-identical.
 #include <algorithm>
 
-#include "rp_kernels.h"
+#include "rp_device.h"
 
 namespace rp {
-
-static __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
-
-// Ordered wave compaction: returns this lane's output slot among lanes with
-// pred set (in lane order) and the wave total in *total.
-static __device__ __forceinline__ int wave_compact(bool pred, int* total) {
-    unsigned long long m = __ballot(pred);
-    *total = __popcll(m);
-    unsigned long long below = (lane_id() == 0) ? 0ull : (m & ((~0ull) >> (64 - lane_id())));
-    return __popcll(below);
-}
 
 // ---------------------------------------------------------------------------
 __global__ void k_frame_begin(DevWorld w, double gravity) {
@@ -53,6 +42,7 @@ __global__ void k_frame_begin(DevWorld w, double gravity) {
 // broad_get_collision_pairs (broad.cpp:6-29): pairs (i<j) in i-major order.
 __global__ void k_broad(DevWorld w) {
     int s = blockIdx.x;
+    if (w.scene_big[s]) return;
     int lane = threadIdx.x;
     int b0 = w.scene_body_off[s], n = w.scene_body_off[s + 1] - b0;
     int base = w.scene_pair_base[s];
@@ -60,17 +50,10 @@ __global__ void k_broad(DevWorld w) {
     for (int i = lane; i < n; i += 64) w.need[b0 + i] = 0;
     for (int i = 0; i < n; ++i) {
         const Body& e1 = w.body[b0 + i];
-        V3 x1 = e1.x;
-        double r1 = e1.radius;
         for (int j0 = i + 1; j0 < n; j0 += 64) {
             int j = j0 + lane;
             bool hit = false;
-            if (j < n) {
-                const Body& e2 = w.body[b0 + j];
-                double dd = len(sub(x1, e2.x));
-                double maxd = r1 + e2.radius + 0.1;
-                hit = dd <= maxd;
-            }
+            if (j < n) hit = broad_hit(e1, w.body[b0 + j]);
             int total;
             int slot = wave_compact(hit, &total);
             if (hit) w.pairs[base + count + slot] = make_int2(i, j);
@@ -89,6 +72,7 @@ static __device__ int uf_find(int* parent, int x) {
 // pairs the narrow phase visits (pbd.cpp:593-596).
 __global__ void k_islands(DevWorld w, double dt) {
     int s = blockIdx.x;
+    if (w.scene_big[s]) return;
     int lane = threadIdx.x;
     int b0 = w.scene_body_off[s], n = w.scene_body_off[s + 1] - b0;
     int base = w.scene_pair_base[s];
@@ -176,6 +160,7 @@ __global__ void k_scan_live(DevWorld w) {
 
 __global__ void k_scatter_live(DevWorld w) {
     int s = blockIdx.x;
+    if (w.scene_big[s]) return;
     int base = w.scene_pair_base[s];
     int g0 = w.live_prefix[s], n = w.live_count[s];
     int b0 = w.scene_body_off[s];
@@ -366,6 +351,7 @@ constexpr int LEV_LDS_BODIES = 1024;
 
 __global__ void __launch_bounds__(64) k_levels(DevWorld w, int collisions) {
     int s = blockIdx.x;
+    if (w.scene_big[s]) return;
     int lane = threadIdx.x;
     int b0 = w.scene_body_off[s], n = w.scene_body_off[s + 1] - b0;
     int e0 = w.scene_ext_off[s], next = w.scene_ext_off[s + 1] - e0;
@@ -438,16 +424,6 @@ __global__ void __launch_bounds__(64) k_levels(DevWorld w, int collisions) {
     }
 }
 
-// Loads of the body fields each solve reads (the rest of Body is unused).
-__device__ __forceinline__ void load_pos_body(const Body& g, Body& b) {
-    b.x = g.x; b.q = g.q; b.px = g.px; b.pq = g.pq;
-    b.inv_mass = g.inv_mass; b.Iinv = g.Iinv; b.mu_s = g.mu_s; b.fixed = g.fixed;
-}
-__device__ __forceinline__ void load_vel_body(const Body& g, Body& b) {
-    b.q = g.q; b.v = g.v; b.w = g.w; b.pv = g.pv; b.pw = g.pw;
-    b.inv_mass = g.inv_mass; b.Iinv = g.Iinv; b.mu_d = g.mu_d; b.e = g.e; b.fixed = g.fixed;
-}
-
 // Solve of one scene and substep: position Gauss-Seidel (pbd.cpp:615-620),
 // velocity update (pbd.cpp:623-643), velocity solve (pbd.cpp:646-711), level
 // by level over the frame's items. LPS lanes serve one scene, so a wave
@@ -458,7 +434,7 @@ __global__ void __launch_bounds__(64) k_solve(DevWorld w, double h, int iters, i
     constexpr int SPW = 64 / LPS;
     int s = blockIdx.x * SPW + (int)threadIdx.x / LPS;
     int lane = (int)threadIdx.x % LPS;
-    bool valid = s < w.ns;
+    bool valid = s < w.ns && !w.scene_big[s];  // large scenes: coloured solve (rp_large.hip)
     int b0 = 0, n = 0, e0 = 0, e1x = 0, L = 0;
     const int* order = nullptr;
     const int* ls = nullptr;
@@ -490,64 +466,17 @@ __global__ void __launch_bounds__(64) k_solve(DevWorld w, double h, int iters, i
     for (int it = 0; it < iters; ++it) {
         for (int l = 0; l < Lmax; ++l) {
             int i0 = (l < L) ? ls[l] : 0, i1 = (l < L) ? ls[l + 1] : 0;
-            for (int idx = i0 + lane; idx < i1; idx += LPS) {
-                int item = order[idx];
-                int bi1, bi2;
-                if (item < 0) { bi1 = w.ext[~item].b1; bi2 = w.ext[~item].b2; }
-                else { int2 p = w.pairs[w.live_global[item].y]; bi1 = b0 + p.x; bi2 = b0 + p.y; }
-                Body B1, B2;
-                load_pos_body(w.body[bi1], B1);
-                load_pos_body(w.body[bi2], B2);
-                if (item < 0) {
-                    if constexpr (kExt) solve_external(w.ext[~item], w.ext_lambda + 3 * (~item), B1, B2, h, err);
-                } else {
-                    int nc = w.ncontact[item];
-                    Contact* cs = w.contacts + (size_t)item * MAXC;
-                    for (int c = 0; c < nc; ++c) {
-                        Contact ct = cs[c];
-                        solve_collision(ct, B1, B2, h, err);
-                        cs[c].lambda_n = ct.lambda_n;
-                        cs[c].lambda_t = ct.lambda_t;
-                    }
-                }
-                if (!B1.fixed) { w.body[bi1].x = B1.x; w.body[bi1].q = B1.q; }
-                if (!B2.fixed) { w.body[bi2].x = B2.x; w.body[bi2].q = B2.q; }
-            }
+            for (int idx = i0 + lane; idx < i1; idx += LPS) solve_item_pos<kExt>(w, order[idx], b0, h, err);
             __syncthreads();
         }
     }
     // velocity update
-    for (int i = lane; i < n; i += LPS) {
-        Body& g = w.body[b0 + i];
-        if (g.fixed || !g.active) continue;
-        Body e;
-        e.x = g.x; e.px = g.px; e.q = g.q; e.pq = g.pq; e.v = g.v; e.w = g.w;
-        e.fixed = 0; e.active = 1;
-        velocity_update(e, h);
-        g.pv = e.pv; g.pw = e.pw; g.v = e.v; g.w = e.w;
-    }
+    for (int i = lane; i < n; i += LPS) velocity_update_body(w.body[b0 + i], h);
     __syncthreads();
     // velocity solve over the collision items, in the same levels
     for (int l = 0; l < Lmax; ++l) {
         int i0 = (l < L) ? ls[l] : 0, i1 = (l < L) ? ls[l + 1] : 0;
-        for (int idx = i0 + lane; idx < i1; idx += LPS) {
-            int item = order[idx];
-            if (item < 0) continue;
-            int nc = w.ncontact[item];
-            if (nc == 0) continue;
-            int2 p = w.pairs[w.live_global[item].y];
-            int bi1 = b0 + p.x, bi2 = b0 + p.y;
-            Body B1, B2;
-            load_vel_body(w.body[bi1], B1);
-            load_vel_body(w.body[bi2], B2);
-            const Contact* cs = w.contacts + (size_t)item * MAXC;
-            for (int c = 0; c < nc; ++c) {
-                Contact ct = cs[c];
-                velocity_solve(ct, B1, B2, h);
-            }
-            if (!B1.fixed) { w.body[bi1].v = B1.v; w.body[bi1].w = B1.w; }
-            if (!B2.fixed) { w.body[bi2].v = B2.v; w.body[bi2].w = B2.w; }
-        }
+        for (int idx = i0 + lane; idx < i1; idx += LPS) solve_item_vel(w, order[idx], b0, h);
         __syncthreads();
     }
     if (err && valid) atomicOr(&w.scene_err[s], err);
@@ -608,17 +537,22 @@ __global__ void k_fp64_selftest(const double* a, const double* b, double* out, i
 // ---------------------------------------------------------------------------
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
-hipError_t launch_frame(const DevWorld& w, const FrameParams& fp, hipStream_t st, KernelTimer* kt) {
+hipError_t launch_frame(const DevWorld& w, const FrameParams& fp, hipStream_t st, KernelTimer* kt,
+                        const BigScene* bigs, int nbig) {
     if (w.nb == 0) return hipSuccess;
     KernelTimer none;
     KernelTimer& t = kt ? *kt : none;
     t.begin(KC_FRAME, st);
     if (!fp.forces_given) hipLaunchKernelGGL(k_frame_begin, dim3(cdiv(w.nb, 256)), dim3(256), 0, st, w, fp.gravity);
     hipLaunchKernelGGL(k_broad, dim3(w.ns), dim3(64), 0, st, w);
+    for (int k = 0; k < nbig; ++k) big_broad(w, bigs[k], st);
     hipLaunchKernelGGL(k_islands, dim3(w.ns), dim3(64), 0, st, w, fp.dt);
+    for (int k = 0; k < nbig; ++k) big_islands(w, bigs[k], fp.dt, st);
     hipLaunchKernelGGL(k_scan_live, dim3(1), dim3(1024), 0, st, w);
     hipLaunchKernelGGL(k_scatter_live, dim3(w.ns), dim3(64), 0, st, w);
+    for (int k = 0; k < nbig; ++k) big_scatter(w, bigs[k], st);
     hipLaunchKernelGGL(k_levels, dim3(w.ns), dim3(64), 0, st, w, fp.collisions ? 1 : 0);
+    for (int k = 0; k < nbig; ++k) big_colour(w, bigs[k], fp.collisions, st);
     t.end(st);
     for (unsigned sub = 0; sub < fp.substeps; ++sub) {
         t.begin(KC_INTEGRATE, st);
@@ -654,6 +588,7 @@ hipError_t launch_frame(const DevWorld& w, const FrameParams& fp, hipStream_t st
         t.begin(KC_SOLVE, st);
         if (w.n_ext > 0) launch_solve<true>(w, fp, st);
         else launch_solve<false>(w, fp, st);
+        for (int k = 0; k < nbig; ++k) big_solve(w, bigs[k], fp, st);
         t.end(st);
     }
     return hipGetLastError();
@@ -686,8 +621,9 @@ int narrow_blocks_for(const DevWorld& w, int g, int64_t pcap) {
     return (int)std::max<int64_t>(1, std::min<int64_t>(blocks, 256 * 16));
 }
 
-hipError_t launch_broad(const DevWorld& w, hipStream_t st) {
+hipError_t launch_broad(const DevWorld& w, hipStream_t st, const BigScene* bigs, int nbig) {
     hipLaunchKernelGGL(k_broad, dim3(w.ns), dim3(64), 0, st, w);
+    for (int k = 0; k < nbig; ++k) big_broad(w, bigs[k], st);
     return hipGetLastError();
 }
 

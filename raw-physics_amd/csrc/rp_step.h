@@ -75,6 +75,7 @@ enum : uint32_t {
     ERR_CLIP_CAPACITY = 1u << 9,
     ERR_CONTACT_CAPACITY = 1u << 10,
     ERR_SOLVER_CAPACITY = 1u << 11,
+    ERR_PAIR_CAPACITY = 1u << 12,
 };
 
 constexpr int EPA_MAXV = 104;  // 4 simplex points + at most 100 expansions (epa.cpp:149)

@@ -240,7 +240,15 @@ static double hull_radius(const HostHull& h) {  // collider.cpp:178-189, 510-521
 // ---------------------------------------------------------------------------
 // Host-side world and its device image
 // ---------------------------------------------------------------------------
+// Scenes above this many bodies take the large-scene path (grid broad
+// phase, parallel islands, graph-coloured solve) unless the desc says
+// otherwise.
+constexpr uint32_t kLargeSceneBodies = 4096;
+// Pair capacity of a large scene per body (plus one per body for each large body).
+constexpr int64_t kLargePairsPerBody = 32;
+
 struct HostWorld {
+    uint32_t large_scene_bodies = kLargeSceneBodies;
     std::vector<int32_t> scene_body_off{0}, scene_ext_off{0};
     std::vector<Body> body;
     std::vector<int32_t> body_hull;
@@ -251,6 +259,7 @@ struct HostWorld {
 
 struct DeviceImage {
     DevWorld d;
+    std::vector<BigScene> big;  // large scenes (device buffers in allocs)
     std::vector<void*> allocs;
     std::vector<int32_t> body_tv_off;  // host copy for inspection
     int64_t pair_capacity = 0;
@@ -332,16 +341,55 @@ static bool build_image(const HostWorld& hw, DeviceImage& im) {
         ntv += (int64_t)h.v.size();
     }
     d.n_xf = (int)xf.size();
-    // pair and constraint capacities per scene
-    std::vector<int32_t> pair_base(d.ns + 1, 0), con_base(d.ns + 1, 0);
+    // pair and constraint capacities per scene; large scenes get a bounded
+    // pair capacity (their broad phase reports ERR_PAIR_CAPACITY beyond it)
+    std::vector<int32_t> pair_base(d.ns + 1, 0), con_base(d.ns + 1, 0), scene_big(d.ns, 0);
+    im.big.clear();
     int64_t pcap = 0, ccap = 0;
     for (int s = 0; s < d.ns; ++s) {
         int64_t n = hw.scene_body_off[s + 1] - hw.scene_body_off[s];
         int64_t next = hw.scene_ext_off[s + 1] - hw.scene_ext_off[s];
         pair_base[s] = (int32_t)pcap;
         con_base[s] = (int32_t)ccap;
-        pcap += n * (n - 1) / 2;
-        ccap += next + n * (n - 1) / 2;
+        int64_t scene_pairs = n * (n - 1) / 2;
+        // (at least 65 bodies: the class starts of the coloured solve need
+        // more item slots than a tiny scene has)
+        if (n > std::max<int64_t>(hw.large_scene_bodies, 64)) {
+            BigScene bs;
+            memset(&bs, 0, sizeof(bs));
+            bs.s = s;
+            bs.b0 = hw.scene_body_off[s];
+            bs.n = (int)n;
+            // bodies far above the typical size (a floor) are tested against
+            // all others; the grid cell bounds every pair of the rest
+            std::vector<double> r(n);
+            for (int64_t i = 0; i < n; ++i) r[i] = hw.body[bs.b0 + i].radius;
+            std::vector<double> sorted_r = r;
+            std::nth_element(sorted_r.begin(), sorted_r.begin() + n / 2, sorted_r.end());
+            double small_bound = 4.0 * sorted_r[n / 2];
+            double rmax = 0.0;
+            int64_t n_large = 0;
+            for (double x : r) {
+                if (x > small_bound) ++n_large;
+                else rmax = std::max(rmax, x);
+            }
+            bs.n_large = (int)n_large;
+            // 1% margin: cell coordinates of two bodies that pass the test
+            // differ by at most one even after rounding
+            bs.cell = (2.0 * rmax + 0.1) * 1.01 + 1e-9;
+            int64_t T = 1024;
+            while (T < 2 * (n - n_large)) T <<= 1;
+            bs.T = (int)T;
+            scene_pairs = std::min(scene_pairs, kLargePairsPerBody * n + n_large * n);
+            bs.pair_base = (int32_t)pcap;
+            bs.pair_cap = (int)scene_pairs;
+            bs.item_base = (int32_t)ccap;
+            bs.item_cap = (int)(next + scene_pairs);
+            im.big.push_back(bs);
+            scene_big[s] = 1;
+        }
+        pcap += scene_pairs;
+        ccap += next + scene_pairs;
         if (pcap > INT32_MAX / MAXC || ccap > INT32_MAX - d.ns - 1) {
             set_msg("world too large for 32-bit pair/constraint indices");
             return false;
@@ -410,6 +458,32 @@ static bool build_image(const HostWorld& hw, DeviceImage& im) {
     if (ok && !hw.scene_err.empty())
         ok &= hipMemcpy(errp, hw.scene_err.data(), d.ns * sizeof(uint32_t), hipMemcpyHostToDevice) == hipSuccess;
     ok &= im.alloc(d.stats, STAT_COUNT);
+    ok &= im.upload(d.scene_big, scene_big);
+    for (BigScene& bs : im.big) {
+        std::vector<int32_t> large, is_large(bs.n, 0);
+        std::vector<double> r(bs.n);
+        for (int i = 0; i < bs.n; ++i) r[i] = hw.body[bs.b0 + i].radius;
+        std::vector<double> sorted_r = r;
+        std::nth_element(sorted_r.begin(), sorted_r.begin() + bs.n / 2, sorted_r.end());
+        double small_bound = 4.0 * sorted_r[bs.n / 2];
+        for (int i = 0; i < bs.n; ++i)
+            if (r[i] > small_bound) { large.push_back(i); is_large[i] = 1; }
+        const int32_t* cp;
+        ok &= im.upload(cp, large); bs.large = const_cast<int32_t*>(cp);
+        ok &= im.upload(cp, is_large); bs.is_large = const_cast<int32_t*>(cp);
+        int64_t tmp_n = std::max<int64_t>(bs.pair_cap, bs.item_cap) + 1;
+        ok &= im.alloc(bs.cell_start, (size_t)bs.T + 1);
+        ok &= im.alloc(bs.cell_cursor, (size_t)bs.T);
+        ok &= im.alloc(bs.sorted, (size_t)bs.n);
+        ok &= im.alloc(bs.pair_off, (size_t)bs.n + 1);
+        ok &= im.alloc(bs.tmp, (size_t)tmp_n);
+        ok &= im.alloc(bs.bi_start, (size_t)bs.n + 1);
+        ok &= im.alloc(bs.bi_cursor, (size_t)bs.n);
+        ok &= im.alloc(bs.bi_items, 2 * (size_t)bs.item_cap);
+        ok &= im.alloc(bs.remaining, 1);
+        int64_t scan_n = std::max<int64_t>({tmp_n, (int64_t)bs.T, (int64_t)bs.n + 1});
+        ok &= im.alloc(bs.scan_blocks, (size_t)scan_block_count(scan_n));
+    }
     im.ok = ok;
     return ok;
 }
@@ -439,6 +513,7 @@ static bool host_world_from_desc(const rp_world_desc* d, HostWorld& hw) {
     uint32_t ns = d->num_scenes;
     uint32_t nb = d->scene_body_offset[ns];
     hw.scene_err.assign(ns, 0);
+    if (d->large_scene_bodies) hw.large_scene_bodies = d->large_scene_bodies;
     // hull instances: one per distinct (shape, scale)
     std::map<std::tuple<uint32_t, uint64_t, uint64_t, uint64_t>, int32_t> inst;
     for (uint32_t s = 0; s < ns; ++s) {
@@ -520,7 +595,12 @@ struct rp_world {
     hipGraphExec_t frame_exec = nullptr;
     FrameParams graph_fp;
     uint64_t sched_gen = 1, graph_gen = 0;  // schedule changes invalidate the captured graph
+    // bounded queue: at most two chunks of frames in flight on the stream
+    hipEvent_t chunk_done[2] = {nullptr, nullptr};
 };
+
+// Frames per chunk of the bounded launch queue.
+constexpr uint32_t kFramesPerChunk = 32;
 
 static bool same_params(const FrameParams& a, const FrameParams& b) {
     return a.dt == b.dt && a.h == b.h && a.gravity == b.gravity && a.substeps == b.substeps && a.iters == b.iters &&
@@ -539,7 +619,7 @@ static hipError_t frame_graph(rp_world* w, const FrameParams& fp) {
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return e;
-    hipError_t le = launch_frame(d, fp, w->stream, nullptr);
+    hipError_t le = launch_frame(d, fp, w->stream, nullptr, w->im.big.data(), (int)w->im.big.size());
     e = hipStreamEndCapture(w->stream, &g);
     if (le != hipSuccess) e = le;
     if (e == hipSuccess) e = hipGraphInstantiate(&w->frame_exec, g, nullptr, nullptr, 0);
@@ -558,6 +638,12 @@ static bool world_init(rp_world* w) {
         set_msg("hipStreamCreate failed");
         return false;
     }
+    for (hipEvent_t& e : w->chunk_done) {
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            set_msg("hipEventCreate failed");
+            return false;
+        }
+    }
     return build_image(w->hw, w->im);
 }
 
@@ -565,16 +651,33 @@ static void world_free(rp_world* w) {
     if (!w) return;
     if (w->stream) (void)hipStreamSynchronize(w->stream);
     if (w->frame_exec) (void)hipGraphExecDestroy(w->frame_exec);
+    for (hipEvent_t e : w->chunk_done)
+        if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : w->timer.pool) (void)hipEventDestroy(e);
     w->im.release();
     if (w->stream) (void)hipStreamDestroy(w->stream);
     delete w;
 }
 
+// Keeps the stream's queue bounded: after each chunk of frames an event is
+// recorded, and the host waits for the chunk before the previous one, so at
+// most two chunks are queued while the device always has one to run.
+static hipError_t chunk_mark(rp_world* w, uint32_t f) {
+    if ((f + 1) % kFramesPerChunk != 0) return hipSuccess;
+    uint32_t k = (f + 1) / kFramesPerChunk;
+    hipError_t e = hipSuccess;
+    if (k >= 2) e = hipEventSynchronize(w->chunk_done[k & 1]);
+    if (e == hipSuccess) e = hipEventRecord(w->chunk_done[k & 1], w->stream);
+    return e;
+}
+
 static uint32_t world_run(rp_world* w, const FrameParams& fp, uint32_t frames, bool sync) {
     if (w->use_graphs && !w->timer.on && frames > 0) {
         hipError_t e = frame_graph(w, fp);
-        for (uint32_t f = 0; e == hipSuccess && f < frames; ++f) e = hipGraphLaunch(w->frame_exec, w->stream);
+        for (uint32_t f = 0; e == hipSuccess && f < frames; ++f) {
+            e = hipGraphLaunch(w->frame_exec, w->stream);
+            if (e == hipSuccess) e = chunk_mark(w, f);
+        }
         if (e != hipSuccess) {
             set_msg(std::string("frame graph: ") + hipGetErrorString(e));
             return RP_ERR_RUNTIME;
@@ -583,7 +686,8 @@ static uint32_t world_run(rp_world* w, const FrameParams& fp, uint32_t frames, b
     }
     for (uint32_t f = 0; f < frames; ++f) {
         w->timer.used = 0;
-        hipError_t e = launch_frame(w->im.d, fp, w->stream, &w->timer);
+        hipError_t e = launch_frame(w->im.d, fp, w->stream, &w->timer, w->im.big.data(), (int)w->im.big.size());
+        if (e == hipSuccess && !w->timer.on) e = chunk_mark(w, f);
         if (e != hipSuccess) {
             set_msg(std::string("kernel launch: ") + hipGetErrorString(e));
             return RP_ERR_RUNTIME;
@@ -738,7 +842,9 @@ void rp_world_body_info(rp_world* w, double* inv_mass, double* inertia, double* 
 uint32_t rp_world_broad_pairs(rp_world* w, uint32_t scene, uint32_t* out, uint32_t cap) {
     const DevWorld& d = w->im.d;
     if ((int)scene >= d.ns) return 0;
-    if (launch_broad(d, w->stream) != hipSuccess || hipStreamSynchronize(w->stream) != hipSuccess) return 0;
+    if (launch_broad(d, w->stream, w->im.big.data(), (int)w->im.big.size()) != hipSuccess ||
+        hipStreamSynchronize(w->stream) != hipSuccess)
+        return 0;
     int32_t n = 0, base = 0;
     (void)hipMemcpy(&n, d.pair_count + scene, 4, hipMemcpyDeviceToHost);
     (void)hipMemcpy(&base, d.scene_pair_base + scene, 4, hipMemcpyDeviceToHost);

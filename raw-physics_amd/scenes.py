@@ -199,7 +199,7 @@ class WorldDescC(ctypes.Structure):
                 ("body_scale", _P), ("body_position", _P), ("body_rotation", _P),
                 ("body_linear_velocity", _P), ("body_angular_velocity", _P), ("body_mass", _P),
                 ("body_fixed", _P), ("body_material", _P), ("scene_constraint_offset", _P),
-                ("constraints", _P)]
+                ("constraints", _P), ("large_scene_bodies", ctypes.c_uint32)]
 
 
 class StateC(ctypes.Structure):
@@ -241,6 +241,7 @@ class World:
     scene_constraint_offset: list = field(default_factory=lambda: [0])
     constraints: list = field(default_factory=list)      # ConstraintDesc
     name: str = ""
+    large_scene_bodies: int = 0                          # device large-scene threshold (0: library default)
 
     # -- building -----------------------------------------------------------
     def shape(self, corners: np.ndarray) -> int:
@@ -317,7 +318,7 @@ class World:
             a["body_linear_velocity"].ctypes.data, a["body_angular_velocity"].ctypes.data,
             a["body_mass"].ctypes.data, a["body_fixed"].ctypes.data, a["body_material"].ctypes.data,
             a["scene_constraint_offset"].ctypes.data if cons is not None else None,
-            ctypes.addressof(cons) if cons is not None else None)
+            ctypes.addressof(cons) if cons is not None else None, self.large_scene_bodies)
         return d, a
 
 
@@ -390,6 +391,57 @@ def batched_uniform(num_scenes: int, first_seed: int = 0, **kw) -> World:
     w = World(name=f"batched_uniform_{num_scenes}")
     for s in range(num_scenes):
         drop_scene(w, first_seed + s, **kw)
+    return w
+
+
+LARGE_SCALES = 64  # scale levels of the large scene (bodies of one level share a hull)
+
+
+def large_scene(n: int, seed: int = 0, large_scene_bodies: int = 0, height: float = 40.0) -> World:
+    """The large single-scene workload: `n` dynamic bodies, 60% cubes and 40%
+    42-vertex icospheres, scale U(0.5, 1.5) (quantised to LARGE_SCALES
+    levels so that bodies share hulls), uniformly random orientation, at rest,
+    seeded uniformly in a (2L) x height x (2L) volume above a static floor
+    (cube.obj x (L + 10, 1, L + 10) at y = -2), L = 100 sqrt(n / 10^5) so the
+    density matches 10^5 bodies in 200 x 40 x 200. Positions are rejection-
+    sampled so that no two bounding spheres overlap. Mass = hull volume
+    (density 1); friction 0.5/0.3, restitution 0."""
+    rng = Rng(0x5EED0000 + seed)
+    L = 100.0 * math.sqrt(n / 1.0e5)
+    w = World(name=f"large_{n}", large_scene_bodies=large_scene_bodies)
+    cube = w.shape(cube_corners())
+    ico = w.shape(icosphere_corners(1))
+    w.add_body(cube, (L + 10.0, 1.0, L + 10.0), (0.0, -2.0, 0.0), quaternion_new((0.0, 1.0, 0.0), 0.0),
+               fixed=True, material=(0.5, 0.3, 0.0))
+    unit_volume = {cube: hull_volume(w.shapes[cube], (1.0, 1.0, 1.0)),
+                   ico: hull_volume(w.shapes[ico], (1.0, 1.0, 1.0))}
+    cell = 2.0 * math.sqrt(3.0) * 1.5
+    grid: dict = {}
+    placed = 0
+    while placed < n:
+        is_cube = rng.uniform(0.0, 1.0) < 0.6
+        level = min(LARGE_SCALES - 1, int(rng.uniform(0.0, LARGE_SCALES)))
+        r = 0.5 + level / (LARGE_SCALES - 1)
+        bound = math.sqrt(3.0) * r if is_cube else r
+        x = rng.uniform(-L, L)
+        y = -1.0 + bound + 0.5 + rng.uniform(0.0, height)
+        z = rng.uniform(-L, L)
+        key = (int(math.floor(x / cell)), int(math.floor(y / cell)), int(math.floor(z / cell)))
+        ok = True
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    for (px, py, pz, pb) in grid.get((key[0] + dx, key[1] + dy, key[2] + dz), ()):
+                        if (px - x) ** 2 + (py - y) ** 2 + (pz - z) ** 2 < (pb + bound) ** 2:
+                            ok = False
+        if not ok:
+            continue
+        grid.setdefault(key, []).append((x, y, z, bound))
+        shape = cube if is_cube else ico
+        w.add_body(shape, (r, r, r), (x, y, z), rng.quaternion(), mass=unit_volume[shape] * r * r * r,
+                   material=(0.5, 0.3, 0.0))
+        placed += 1
+    w.end_scene()
     return w
 
 
