@@ -222,6 +222,7 @@ enum {
     RP_ERR_CONTACT_CAPACITY = 1u << 10, /* more than 16 contacts for one body pair */
     RP_ERR_SOLVER_CAPACITY = 1u << 11,  /* constraint list exceeded its capacity */
     RP_ERR_PAIR_CAPACITY = 1u << 12,    /* broad-phase pairs of a large scene exceeded its capacity */
+    RP_ERR_ADJOINT_SCOPE = 1u << 13,    /* adjoint on a scene outside its scope (one dynamic body) */
     RP_ERR_RUNTIME = 1u << 31         /* HIP runtime failure (see rp_last_error_string) */
 };
 
@@ -255,6 +256,30 @@ uint64_t rp_world_epa_failures(rp_world* w);
 uint64_t rp_world_gjk_cycle_exits(rp_world* w);
 /* The HIP stream the world's kernels run on (a hipStream_t). */
 void* rp_world_stream(rp_world* w);
+
+/* ---- Gradients (reverse mode over a recorded run) --------------------------
+ * Scope: every scene has one dynamic body (any number of fixed bodies), no
+ * external constraints, and is not a large scene; otherwise
+ * RP_ERR_ADJOINT_SCOPE. The contact set is frozen: the contacts the forward
+ * pass generated (r1_lc, r2_lc, n per substep) are constants of the
+ * gradient. No counterpart in the reference, which has no gradients. */
+
+/* Mass (1/m, inertia and its inverse as at creation), static and dynamic
+ * friction of every non-fixed body; NULL arrays are left unchanged. */
+uint32_t rp_world_set_body_params(rp_world* w, const double* mass, const double* mu_s, const double* mu_d);
+/* Steps `frames` frames (collisions on) from the current state, keeping a
+ * device checkpoint of every frame's start state. */
+uint32_t rp_world_adjoint_forward(rp_world* w, double dt, uint32_t frames, uint32_t num_substeps,
+                                  uint32_t num_pos_iters, double gravity);
+/* dl_dstate [nb][13]: dL/d(x, q(x,y,z,w), v, w) of the state at the end of the
+ * recorded run. Out: grad_state [nb][13] (the same at its start) and
+ * grad_param [nb][3] (dL/d mass, static friction, dynamic friction). The
+ * world is left at the end state. Each frame is recomputed from its
+ * checkpoint, then its substeps are reversed. */
+uint32_t rp_world_adjoint_backward(rp_world* w, const double* dl_dstate, double* grad_state, double* grad_param);
+/* Substeps whose recomputation in the backward pass did not reproduce the
+ * recorded forward state bit for bit (0 when the backward pass is exact). */
+uint64_t rp_world_adjoint_mismatches(rp_world* w);
 
 /* Narrow-phase schedules: lanes cooperating on one body pair in the GJK + EPA
  * kernel (narrow group) and in the contact-manifold kernel (manifold group):

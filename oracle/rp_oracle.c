@@ -19,10 +19,15 @@
  * compiles this file as C++ with `real` a double wrapper that counts every
  * add, subtract, multiply, divide and square root, attributed to the phase
  * named by ORC_PHASE. RD() reads the double value of a real. */
-#ifndef ORC_COUNT_OPS
+#if !defined(ORC_COUNT_OPS) && !defined(ORC_DUAL)
 typedef double real;
 #define RD(x) (x)
 #define ORC_PHASE(p) ((void)0)
+#endif
+/* The dual-number build (dual.cpp) strips derivatives of the contact geometry
+ * here when it differentiates with the contact set frozen. */
+#ifndef ORC_FREEZE_V3
+#define ORC_FREEZE_V3(v) ((void)0)
 #endif
 enum { PH_FRAME = 0, PH_INTEGRATE, PH_TRANSFORM, PH_NARROW, PH_POSITION, PH_VELOCITY_UPDATE, PH_VELOCITY_SOLVE, PH_SETUP,
        PH_MANIFOLD, PH_COUNT };
@@ -286,6 +291,16 @@ typedef struct {
     uint32_t err;
 } scene;
 
+/* Contact log: in record mode every narrow-phase visit appends its pair and
+ * collision constraints (values); in replay mode the narrow phase is skipped
+ * and the logged constraints are used in the same order. Replaying runs the
+ * step with the contact set (and contact geometry) of the recorded run, the
+ * map whose derivatives the device adjoint computes. */
+typedef struct { uint32_t i1, i2, n; } log_pair;
+typedef struct { double r1[3], r2[3], n[3]; } log_contact;
+typedef ARR(log_pair) logpairarr;
+typedef ARR(log_contact) logcontactarr;
+
 struct orc_world {
     uint32_t ns;
     scene* s;
@@ -293,6 +308,11 @@ struct orc_world {
     uint64_t epa_failures;
     uint64_t last_contacts;
     uint64_t last_narrow_pairs;
+    int log_mode;                 /* 0 off, 1 record, 2 replay */
+    logpairarr log_pairs;
+    logcontactarr log_contacts;
+    uint32_t log_pair_cur, log_contact_cur;
+    int log_error;
 };
 
 /* ------------------------------------------------------------------ */
@@ -1311,6 +1331,9 @@ static constraint contact_to_constraint(const body* e1, const body* e2, uint32_t
     c.r1 = q_rot(q1i, r1w);
     qt q2i = q_inv(e2->q);
     c.r2 = q_rot(q2i, r2w);
+    ORC_FREEZE_V3(c.n);
+    ORC_FREEZE_V3(c.r1);
+    ORC_FREEZE_V3(c.r2);
     return c;
 }
 
@@ -1445,11 +1468,42 @@ static void simulate(orc_world* w, scene* sc, real dt, uint32_t substeps, uint32
                     sc->err |= ORC_ERR_ACTIVITY;
                 if ((e1->fixed || !e1->active) && (e2->fixed || !e2->active)) continue;
                 ++w->last_narrow_pairs;
+                if (w->log_mode == 2) {  /* replay the logged constraints of this visit */
+                    if (w->log_pair_cur >= w->log_pairs.n || w->log_pairs.d[w->log_pair_cur].i1 != i1 ||
+                        w->log_pairs.d[w->log_pair_cur].i2 != i2) {
+                        w->log_error = 1;
+                        continue;
+                    }
+                    uint32_t nl = w->log_pairs.d[w->log_pair_cur++].n;
+                    for (uint32_t l = 0; l < nl; ++l) {
+                        const log_contact* lc = &w->log_contacts.d[w->log_contact_cur++];
+                        constraint c;
+                        memset(&c, 0, sizeof(c));
+                        c.type = ORC_COLLISION_CONSTRAINT;
+                        c.b1 = i1; c.b2 = i2;
+                        c.r1 = V(lc->r1[0], lc->r1[1], lc->r1[2]);
+                        c.r2 = V(lc->r2[0], lc->r2[1], lc->r2[2]);
+                        c.n = V(lc->n[0], lc->n[1], lc->n[2]);
+                        c.lambda_n = 0.0; c.lambda_t = 0.0;
+                        ARR_PUSH(cons, c);
+                    }
+                    w->last_contacts += nl;
+                    continue;
+                }
                 cts.n = 0;
                 pair_contacts(&e1->h, &e2->h, &cts, &sc->err, &w->epa_failures);
                 for (uint32_t l = 0; l < cts.n; ++l) {
                     constraint c = contact_to_constraint(e1, e2, i1, i2, &cts.d[l]);
                     ARR_PUSH(cons, c);
+                    if (w->log_mode == 1) {
+                        log_contact lc = {{RD(c.r1.x), RD(c.r1.y), RD(c.r1.z)}, {RD(c.r2.x), RD(c.r2.y), RD(c.r2.z)},
+                                          {RD(c.n.x), RD(c.n.y), RD(c.n.z)}};
+                        ARR_PUSH(w->log_contacts, lc);
+                    }
+                }
+                if (w->log_mode == 1) {
+                    log_pair lp = {i1, i2, cts.n};
+                    ARR_PUSH(w->log_pairs, lp);
                 }
                 w->last_contacts += cts.n;
             }
@@ -1593,8 +1647,31 @@ orc_world* orc_world_create(const orc_world_desc* d) {
     return w;
 }
 
+void orc_world_contact_log(orc_world* w, int mode) {
+    if (mode == 1) {
+        w->log_pairs.n = 0;
+        w->log_contacts.n = 0;
+    }
+    w->log_mode = mode;
+    w->log_pair_cur = w->log_contact_cur = 0;
+    w->log_error = 0;
+}
+
+int orc_world_contact_log_error(const orc_world* w) { return w->log_error; }
+
+/* Copies src's contact log into dst (for replay in a world built with other
+ * parameters). */
+void orc_world_contact_log_copy(orc_world* dst, const orc_world* src) {
+    dst->log_pairs.n = 0;
+    dst->log_contacts.n = 0;
+    for (uint32_t k = 0; k < src->log_pairs.n; ++k) ARR_PUSH(dst->log_pairs, src->log_pairs.d[k]);
+    for (uint32_t k = 0; k < src->log_contacts.n; ++k) ARR_PUSH(dst->log_contacts, src->log_contacts.d[k]);
+}
+
 void orc_world_destroy(orc_world* w) {
     if (!w) return;
+    free(w->log_pairs.d);
+    free(w->log_contacts.d);
     for (uint32_t si = 0; si < w->ns; ++si) {
         for (uint32_t k = 0; k < w->s[si].nb; ++k) hull_free(&w->s[si].b[k].h);
         free(w->s[si].b);

@@ -104,6 +104,12 @@ enum {
 
 orc_world* orc_world_create(const orc_world_desc* desc);
 void orc_world_destroy(orc_world* w);
+/* Contact log (see rp_oracle.c): 1 = record from now, 2 = replay from the
+ * start of the log, 0 = off. The error flag is set when a replayed visit
+ * does not match the logged pair. */
+void orc_world_contact_log(orc_world* w, int mode);
+int orc_world_contact_log_error(const orc_world* w);
+void orc_world_contact_log_copy(orc_world* dst, const orc_world* src);
 uint32_t orc_world_num_bodies(const orc_world* w);
 
 /* Runs `frames` frames of the reference's caller pattern on every scene

@@ -53,6 +53,10 @@ def library():
         "physics_simulate": (None, [_P, f64]),
         "rp_fp64_selftest": (u32, [_P, _P, _P, u64]),
         "rp_world_set_profiling": (None, [_P, i32]),
+        "rp_world_set_body_params": (u32, [_P, _P, _P, _P]),
+        "rp_world_adjoint_forward": (u32, [_P, f64, u32, u32, u32, f64]),
+        "rp_world_adjoint_backward": (u32, [_P, _P, _P, _P]),
+        "rp_world_adjoint_mismatches": (u64, [_P]),
         "rp_world_set_narrow_group": (i32, [_P, i32]),
         "rp_world_narrow_group": (i32, [_P]),
         "rp_world_set_manifold_group": (i32, [_P, i32]),
@@ -124,6 +128,32 @@ class World:
 
     def epa_failures(self) -> int:
         return int(self.lib.rp_world_epa_failures(self.h))
+
+    # -- gradients ------------------------------------------------------------
+    def set_body_params(self, mass=None, mu_s=None, mu_d=None) -> int:
+        """Per-body mass and friction (arrays of nb; None keeps the current)."""
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (mass, mu_s, mu_d)]
+        ptrs = [None if a is None else a.ctypes.data for a in arrs]
+        return int(self.lib.rp_world_set_body_params(self.h, *ptrs))
+
+    def adjoint_forward(self, frames: int, dt=1.0 / 60.0, substeps=20, iters=1, gravity=10.0) -> int:
+        """Steps `frames` frames keeping per-frame checkpoints for adjoint_backward."""
+        return int(self.lib.rp_world_adjoint_forward(self.h, dt, frames, substeps, iters, gravity))
+
+    def adjoint_backward(self, dl_dstate: np.ndarray):
+        """dl_dstate (nb, 13): dL/d(x, q, v, w) at the end of the recorded run.
+        Returns (grad_state (nb, 13) at its start, grad_param (nb, 3) for mass,
+        static and dynamic friction)."""
+        dl = np.ascontiguousarray(dl_dstate, dtype=np.float64).reshape(self.nb, 13)
+        gs = np.zeros((self.nb, 13))
+        gp = np.zeros((self.nb, 3))
+        err = self.lib.rp_world_adjoint_backward(self.h, dl.ctypes.data, gs.ctypes.data, gp.ctypes.data)
+        if err:
+            raise RuntimeError(f"adjoint_backward failed ({err:#x}): {last_error_string()}")
+        return gs, gp
+
+    def adjoint_mismatches(self) -> int:
+        return int(self.lib.rp_world_adjoint_mismatches(self.h))
 
     def gjk_cycle_exits(self) -> int:
         return int(self.lib.rp_world_gjk_cycle_exits(self.h))

@@ -6,7 +6,7 @@
 
 namespace rp {
 
-enum { STAT_EPA_FAIL = 0, STAT_GJK_CYCLE = 1, STAT_COUNT = 4 };
+enum { STAT_EPA_FAIL = 0, STAT_GJK_CYCLE = 1, STAT_ADJ_MISMATCH = 2, STAT_COUNT = 4 };
 enum { CNT_MANIFOLD = 0, CNT_COUNT = 4 };
 
 // All device pointers of one world. Passed by value to every kernel.
@@ -94,6 +94,16 @@ struct BigScene {
     int32_t* scan_blocks;        // [scan block sums]
 };
 
+// Substep records of one frame for the adjoint's recomputation: body
+// states at the start of each substep and the contacts the narrow phase made
+// (the frozen contact set).
+struct Recorder {
+    Body* body;          // [substeps][nb]
+    int32_t* ncontact;   // [substeps][pair capacity]
+    Contact* contacts;   // [substeps][pair capacity * MAXC]
+    int64_t pair_cap;
+};
+
 struct FrameParams {
     double dt = 0.0, h = 0.0, gravity = 0.0;
     unsigned substeps = 0, iters = 0;
@@ -132,7 +142,13 @@ struct KernelTimer {
 };
 
 hipError_t launch_frame(const DevWorld& w, const FrameParams& fp, hipStream_t st, KernelTimer* kt = nullptr,
-                        const BigScene* bigs = nullptr, int nbig = 0);
+                        const BigScene* bigs = nullptr, int nbig = 0, const Recorder* rec = nullptr);
+// Adjoint of one recorded frame for scenes with one dynamic body
+// (rp_adjoint.hip): abar [nb][13] is the adjoint of (x, q, v, w) at the end
+// of the frame on entry and at its start on exit; gparam [nb][3] accumulates
+// d/d(mass, static friction, dynamic friction).
+hipError_t launch_adjoint_frame(const DevWorld& w, const Recorder& rec, const int32_t* scene_dyn, const Body* end_state,
+                                double* abar, double* gparam, const FrameParams& fp, hipStream_t st);
 // Large-scene stages (rp_large.hip), in frame order.
 void big_broad(const DevWorld& w, const BigScene& b, hipStream_t st);
 void big_islands(const DevWorld& w, const BigScene& b, double dt, hipStream_t st);

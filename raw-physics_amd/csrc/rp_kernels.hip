@@ -537,8 +537,24 @@ __global__ void k_fp64_selftest(const double* a, const double* b, double* out, i
 // ---------------------------------------------------------------------------
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
+// Recording for the adjoint: body states at the start of substep k, and the
+// substep's contacts after the narrow phase.
+__global__ void k_rec_bodies(DevWorld w, Recorder r, int k) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < w.nb) r.body[(size_t)k * w.nb + b] = w.body[b];
+}
+__global__ void k_rec_contacts(DevWorld w, Recorder r, int k) {
+    int total = w.live_prefix[w.ns];
+    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += gridDim.x * blockDim.x) {
+        int nc = w.ncontact[g];
+        r.ncontact[(size_t)k * r.pair_cap + g] = nc;
+        for (int c = 0; c < nc; ++c)
+            r.contacts[((size_t)k * r.pair_cap + g) * MAXC + c] = w.contacts[(size_t)g * MAXC + c];
+    }
+}
+
 hipError_t launch_frame(const DevWorld& w, const FrameParams& fp, hipStream_t st, KernelTimer* kt,
-                        const BigScene* bigs, int nbig) {
+                        const BigScene* bigs, int nbig, const Recorder* rec) {
     if (w.nb == 0) return hipSuccess;
     KernelTimer none;
     KernelTimer& t = kt ? *kt : none;
@@ -555,6 +571,7 @@ hipError_t launch_frame(const DevWorld& w, const FrameParams& fp, hipStream_t st
     for (int k = 0; k < nbig; ++k) big_colour(w, bigs[k], fp.collisions, st);
     t.end(st);
     for (unsigned sub = 0; sub < fp.substeps; ++sub) {
+        if (rec) hipLaunchKernelGGL(k_rec_bodies, dim3(cdiv(w.nb, 256)), dim3(256), 0, st, w, *rec, (int)sub);
         t.begin(KC_INTEGRATE, st);
         hipLaunchKernelGGL(k_integrate, dim3(cdiv(w.nb, 128)), dim3(128), 0, st, w, fp.h);
         t.end(st);
@@ -584,6 +601,7 @@ hipError_t launch_frame(const DevWorld& w, const FrameParams& fp, hipStream_t st
             default: hipLaunchKernelGGL(k_manifold<1>, mb, t64, 0, st, w); break;
             }
             t.end(st);
+            if (rec) hipLaunchKernelGGL(k_rec_contacts, dim3(256), dim3(256), 0, st, w, *rec, (int)sub);
         }
         t.begin(KC_SOLVE, st);
         if (w.n_ext > 0) launch_solve<true>(w, fp, st);

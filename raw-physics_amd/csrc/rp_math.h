@@ -13,34 +13,42 @@
 
 namespace rp {
 
-struct V3 { double x, y, z; };
-struct Q4 { double x, y, z, w; };
-struct M3 { double m[3][3]; };
+// Value types over a scalar T: double for the step; the adjoint evaluates
+// the solver functions with a dual-number scalar (rp_dual.h). The double
+// instantiations perform exactly the operations written here.
+template <class T> struct V3T { T x, y, z; };
+template <class T> struct Q4T { T x, y, z, w; };
+template <class T> struct M3T { T m[3][3]; };
+using V3 = V3T<double>;
+using Q4 = Q4T<double>;
+using M3 = M3T<double>;
 struct M4 { double m[4][4]; };
 
-RP_HD V3 v3(double x, double y, double z) { V3 r; r.x = x; r.y = y; r.z = z; return r; }
+template <class T> RP_HD V3T<T> v3(T x, T y, T z) { V3T<T> r; r.x = x; r.y = y; r.z = z; return r; }
 // gm.h:667-694, 722-729
-RP_HD V3 add(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
-RP_HD V3 sub(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
-RP_HD V3 scl(double s, V3 v) { return v3(s * v.x, s * v.y, s * v.z); }
-RP_HD double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-RP_HD V3 cross(V3 a, V3 b) { return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+template <class T> RP_HD V3T<T> add(V3T<T> a, V3T<T> b) { return v3<T>(a.x + b.x, a.y + b.y, a.z + b.z); }
+template <class T> RP_HD V3T<T> sub(V3T<T> a, V3T<T> b) { return v3<T>(a.x - b.x, a.y - b.y, a.z - b.z); }
+template <class S, class T> RP_HD V3T<T> scl(S s, V3T<T> v) { return v3<T>(s * v.x, s * v.y, s * v.z); }
+template <class T> RP_HD T dot(V3T<T> a, V3T<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class T> RP_HD V3T<T> cross(V3T<T> a, V3T<T> b) {
+    return v3<T>(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
 // gm.h:649-652
-RP_HD double len(V3 v) { return sqrt(v.x * v.x + v.y * v.y + v.z * v.z); }
+template <class T> RP_HD T len(V3T<T> v) { return sqrt(v.x * v.x + v.y * v.y + v.z * v.z); }
 // gm.h:627-634: zero stays zero
-RP_HD V3 nrm(V3 v) {
-    if (!(v.x != 0.0 || v.y != 0.0 || v.z != 0.0)) return v3(0.0, 0.0, 0.0);
-    double l = len(v);
-    return v3(v.x / l, v.y / l, v.z / l);
+template <class T> RP_HD V3T<T> nrm(V3T<T> v) {
+    if (!(v.x != 0.0 || v.y != 0.0 || v.z != 0.0)) return v3<T>(T(0.0), T(0.0), T(0.0));
+    T l = len(v);
+    return v3<T>(v.x / l, v.y / l, v.z / l);
 }
 // gm.h:588-590: 0 - v
-RP_HD V3 inv3(V3 v) { return sub(v3(0.0, 0.0, 0.0), v); }
+template <class T> RP_HD V3T<T> inv3(V3T<T> v) { return sub(v3<T>(T(0.0), T(0.0), T(0.0)), v); }
 RP_HD bool veq(V3 a, V3 b) { return a.x == b.x && a.y == b.y && a.z == b.z; }
 RP_HD bool is_zero(V3 v) { return !(v.x != 0.0 || v.y != 0.0 || v.z != 0.0); }
 
 // gm.h:370-381
-RP_HD M3 m3_mul(const M3& a, const M3& b) {
-    M3 r;
+template <class T> RP_HD M3T<T> m3_mul(const M3T<T>& a, const M3T<T>& b) {
+    M3T<T> r;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -48,13 +56,13 @@ RP_HD M3 m3_mul(const M3& a, const M3& b) {
     return r;
 }
 // gm.h:420-426
-RP_HD V3 m3_mulv(const M3& m, V3 v) {
-    return v3(m.m[0][0] * v.x + m.m[0][1] * v.y + m.m[0][2] * v.z, m.m[1][0] * v.x + m.m[1][1] * v.y + m.m[1][2] * v.z,
-              m.m[2][0] * v.x + m.m[2][1] * v.y + m.m[2][2] * v.z);
+template <class T> RP_HD V3T<T> m3_mulv(const M3T<T>& m, V3T<T> v) {
+    return v3<T>(m.m[0][0] * v.x + m.m[0][1] * v.y + m.m[0][2] * v.z, m.m[1][0] * v.x + m.m[1][1] * v.y + m.m[1][2] * v.z,
+                 m.m[2][0] * v.x + m.m[2][1] * v.y + m.m[2][2] * v.z);
 }
 // gm.h:445-452
-RP_HD M3 m3_t(const M3& m) {
-    M3 r;
+template <class T> RP_HD M3T<T> m3_t(const M3T<T>& m) {
+    M3T<T> r;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -73,8 +81,8 @@ RP_HD M4 m4_mul(const M4& a, const M4& b) {
 }
 
 // quaternion.cpp:133-142
-RP_HD Q4 q_mul(Q4 a, Q4 b) {
-    Q4 r;
+template <class T> RP_HD Q4T<T> q_mul(Q4T<T> a, Q4T<T> b) {
+    Q4T<T> r;
     r.w = a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z;
     r.x = a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y;
     r.y = a.w * b.y + a.y * b.w + a.z * b.x - a.x * b.z;
@@ -82,30 +90,30 @@ RP_HD Q4 q_mul(Q4 a, Q4 b) {
     return r;
 }
 // quaternion.cpp:144-147
-RP_HD Q4 q_nrm(Q4 q) {
-    double l = sqrt(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
-    Q4 r;
+template <class T> RP_HD Q4T<T> q_nrm(Q4T<T> q) {
+    T l = sqrt(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
+    Q4T<T> r;
     r.x = q.x / l; r.y = q.y / l; r.z = q.z / l; r.w = q.w / l;
     return r;
 }
 // quaternion.cpp:80-87
-RP_HD Q4 q_inv(Q4 q) {
-    Q4 r;
+template <class T> RP_HD Q4T<T> q_inv(Q4T<T> q) {
+    Q4T<T> r;
     r.x = -q.x; r.y = -q.y; r.z = -q.z; r.w = q.w;
     return r;
 }
 // quaternion.cpp:257-268
-RP_HD V3 q_rot(Q4 q, V3 v) {
-    double ix = q.w * v.x + q.y * v.z - q.z * v.y;
-    double iy = q.w * v.y + q.z * v.x - q.x * v.z;
-    double iz = q.w * v.z + q.x * v.y - q.y * v.x;
-    double iw = -q.x * v.x - q.y * v.y - q.z * v.z;
-    return v3((ix * q.w) + (iw * -q.x) + (iy * -q.z) - (iz * -q.y), (iy * q.w) + (iw * -q.y) + (iz * -q.x) - (ix * -q.z),
+template <class T> RP_HD V3T<T> q_rot(Q4T<T> q, V3T<T> v) {
+    T ix = q.w * v.x + q.y * v.z - q.z * v.y;
+    T iy = q.w * v.y + q.z * v.x - q.x * v.z;
+    T iz = q.w * v.z + q.x * v.y - q.y * v.x;
+    T iw = -q.x * v.x - q.y * v.y - q.z * v.z;
+    return v3<T>((ix * q.w) + (iw * -q.x) + (iy * -q.z) - (iz * -q.y), (iy * q.w) + (iw * -q.y) + (iz * -q.x) - (ix * -q.z),
               (iz * q.w) + (iw * -q.z) + (ix * -q.y) - (iy * -q.x));
 }
 // quaternion.cpp:89-105
-RP_HD M3 q_mat3(Q4 q) {
-    M3 r;
+template <class T> RP_HD M3T<T> q_mat3(Q4T<T> q) {
+    M3T<T> r;
     r.m[0][0] = 1.0 - 2.0 * q.y * q.y - 2.0 * q.z * q.z;
     r.m[1][0] = 2.0 * q.x * q.y + 2.0 * q.w * q.z;
     r.m[2][0] = 2.0 * q.x * q.z - 2.0 * q.w * q.y;
@@ -158,10 +166,10 @@ RP_HD V3 q_axis(Q4 q, int axis, bool& bad) {
 }
 
 // Dynamic (world) inverse inertia R * I^-1 * R^T (physics_util.cpp:44-61)
-RP_HD M3 world_tensor(Q4 q, const M3& local) {
-    M3 R = q_mat3(q);
-    M3 Rt = m3_t(R);
-    M3 aux = m3_mul(R, local);
+template <class T> RP_HD M3T<T> world_tensor(Q4T<T> q, const M3T<T>& local) {
+    M3T<T> R = q_mat3(q);
+    M3T<T> Rt = m3_t(R);
+    M3T<T> aux = m3_mul(R, local);
     return m3_mul(aux, Rt);
 }
 

@@ -14,23 +14,25 @@ namespace rp {
 // ---------------------------------------------------------------------------
 // Device data layout
 // ---------------------------------------------------------------------------
-struct Body {            // one record per body, 464 bytes
-    V3 x; Q4 q; V3 v; V3 w;      // world_position, world_rotation, velocities
-    V3 px; Q4 pq; V3 pv; V3 pw;  // previous_* (entity.h:43-46)
-    double deact;                // deactivation_time
-    double inv_mass;
-    M3 I, Iinv;                  // local inertia tensor and its inverse
-    double mu_s, mu_d, e, radius;
-    V3 F, T;                     // external force / torque of the frame
+template <class S> struct BodyT {   // one record per body (464 bytes for S = double)
+    V3T<S> x; Q4T<S> q; V3T<S> v; V3T<S> w;      // world_position, world_rotation, velocities
+    V3T<S> px; Q4T<S> pq; V3T<S> pv; V3T<S> pw;  // previous_* (entity.h:43-46)
+    S deact;                                     // deactivation_time
+    S inv_mass;
+    M3T<S> I, Iinv;                              // local inertia tensor and its inverse
+    S mu_s, mu_d, e, radius;
+    V3T<S> F, T;                                 // external force / torque of the frame
     int32_t fixed, active;
 };
+using Body = BodyT<double>;
 
 // A collision constraint as built by clipping_contact_to_collision_constraint
 // (pbd.cpp:408-424) plus its multipliers.
-struct Contact {
-    V3 r1, r2, n;
-    double lambda_n, lambda_t;
+template <class S> struct ContactT {
+    V3T<S> r1, r2, n;
+    S lambda_n, lambda_t;
 };
+using Contact = ContactT<double>;
 
 // External constraint (Constraint, pbd.h:22-91); b1/b2 are global body indices.
 struct ExtC {
@@ -76,6 +78,7 @@ enum : uint32_t {
     ERR_CONTACT_CAPACITY = 1u << 10,
     ERR_SOLVER_CAPACITY = 1u << 11,
     ERR_PAIR_CAPACITY = 1u << 12,
+    ERR_ADJOINT_SCOPE = 1u << 13,
 };
 
 constexpr int EPA_MAXV = 104;  // 4 simplex points + at most 100 expansions (epa.cpp:149)
@@ -611,28 +614,32 @@ RP_HD int pair_contacts(const HullView& h1, const HullView& h2, ContactPt* out, 
 // ---------------------------------------------------------------------------
 // XPBD base constraints (pbd_base_constraints.cpp:6-227)
 // ---------------------------------------------------------------------------
-struct PosPrep { V3 r1, r2; M3 I1, I2; };
+template <class S> struct PosPrepT { V3T<S> r1, r2; M3T<S> I1, I2; };
+using PosPrep = PosPrepT<double>;
 
-RP_HD void pos_prep(const Body& e1, const Body& e2, V3 r1_lc, V3 r2_lc, PosPrep& p) {
+template <class S>
+RP_HD void pos_prep(const BodyT<S>& e1, const BodyT<S>& e2, V3T<S> r1_lc, V3T<S> r2_lc, PosPrepT<S>& p) {
     p.r1 = q_rot(e1.q, r1_lc);
     p.r2 = q_rot(e2.q, r2_lc);
     p.I1 = world_tensor(e1.q, e1.Iinv);
     p.I2 = world_tensor(e2.q, e2.Iinv);
 }
-RP_HD double pos_dlambda(const Body& e1, const Body& e2, const PosPrep& p, double h, double compliance, double lambda,
-                         V3 dx, uint32_t& err) {
-    double c = len(dx);
-    if (c <= 1e-50) return 0.0;
-    V3 n = v3(dx.x / c, dx.y / c, dx.z / c);
-    double w1 = e1.inv_mass + dot(cross(p.r1, n), m3_mulv(p.I1, cross(p.r1, n)));
-    double w2 = e2.inv_mass + dot(cross(p.r2, n), m3_mulv(p.I2, cross(p.r2, n)));
+template <class S>
+RP_HD S pos_dlambda(const BodyT<S>& e1, const BodyT<S>& e2, const PosPrepT<S>& p, double h, double compliance, S lambda,
+                    V3T<S> dx, uint32_t& err) {
+    S c = len(dx);
+    if (c <= 1e-50) return S(0.0);
+    V3T<S> n = v3<S>(dx.x / c, dx.y / c, dx.z / c);
+    S w1 = e1.inv_mass + dot(cross(p.r1, n), m3_mulv(p.I1, cross(p.r1, n)));
+    S w2 = e2.inv_mass + dot(cross(p.r2, n), m3_mulv(p.I2, cross(p.r2, n)));
     if (w1 + w2 == 0.0) err |= ERR_ZERO_MASS;
     double tc = compliance / (h * h);
     return (-c - tc * lambda) / (w1 + w2 + tc);
 }
-RP_HD void rot_update(Body& e, V3 a, bool plus) {
-    Q4 aq = {a.x, a.y, a.z, 0.0};
-    Q4 qq = q_mul(aq, e.q);
+template <class S>
+RP_HD void rot_update(BodyT<S>& e, V3T<S> a, bool plus) {
+    Q4T<S> aq = {a.x, a.y, a.z, S(0.0)};
+    Q4T<S> qq = q_mul(aq, e.q);
     if (plus) {
         e.q.x = e.q.x + 0.5 * qq.x; e.q.y = e.q.y + 0.5 * qq.y; e.q.z = e.q.z + 0.5 * qq.z; e.q.w = e.q.w + 0.5 * qq.w;
     } else {
@@ -640,15 +647,16 @@ RP_HD void rot_update(Body& e, V3 a, bool plus) {
     }
     e.q = q_nrm(e.q);
 }
-RP_HD void pos_apply(Body& e1, Body& e2, const PosPrep& p, double dl, V3 dx) {
-    double c = len(dx);
+template <class S>
+RP_HD void pos_apply(BodyT<S>& e1, BodyT<S>& e2, const PosPrepT<S>& p, S dl, V3T<S> dx) {
+    S c = len(dx);
     if (c <= 1e-50) return;
-    V3 n = v3(dx.x / c, dx.y / c, dx.z / c);
-    V3 imp = scl(dl, n);
+    V3T<S> n = v3<S>(dx.x / c, dx.y / c, dx.z / c);
+    V3T<S> imp = scl(dl, n);
     if (!e1.fixed) e1.x = add(e1.x, scl(e1.inv_mass, imp));
     if (!e2.fixed) e2.x = add(e2.x, scl(-e2.inv_mass, imp));
-    V3 a1 = m3_mulv(p.I1, cross(p.r1, imp));
-    V3 a2 = m3_mulv(p.I2, cross(p.r2, imp));
+    V3T<S> a1 = m3_mulv(p.I1, cross(p.r1, imp));
+    V3T<S> a2 = m3_mulv(p.I2, cross(p.r2, imp));
     // quaternion products use the rotation before either body is updated
     // (pbd_base_constraints.cpp:114-117); e1 and e2 are distinct bodies.
     if (!e1.fixed) rot_update(e1, a1, true);
@@ -676,29 +684,30 @@ RP_HD void ang_apply(Body& e1, Body& e2, const M3& I1, const M3& I2, double dl, 
 }
 
 // collision_constraint_solve (pbd.cpp:107-154)
-RP_HD void solve_collision(Contact& c, Body& e1, Body& e2, double h, uint32_t& err) {
-    PosPrep p;
+template <class S>
+RP_HD void solve_collision(ContactT<S>& c, BodyT<S>& e1, BodyT<S>& e2, double h, uint32_t& err) {
+    PosPrepT<S> p;
     pos_prep(e1, e2, c.r1, c.r2, p);
-    V3 p1 = add(e1.x, p.r1);
-    V3 p2 = add(e2.x, p.r2);
-    double d = dot(sub(p1, p2), c.n);
+    V3T<S> p1 = add(e1.x, p.r1);
+    V3T<S> p2 = add(e2.x, p.r2);
+    S d = dot(sub(p1, p2), c.n);
     if (d > 0.0) {
-        V3 dx = scl(d, c.n);
-        double dl = pos_dlambda(e1, e2, p, h, 0.0, c.lambda_n, dx, err);
+        V3T<S> dx = scl(d, c.n);
+        S dl = pos_dlambda(e1, e2, p, h, 0.0, c.lambda_n, dx, err);
         pos_apply(e1, e2, p, dl, dx);
         c.lambda_n += dl;
         pos_prep(e1, e2, c.r1, c.r2, p);
         p1 = add(e1.x, p.r1);
         p2 = add(e2.x, p.r2);
         dl = pos_dlambda(e1, e2, p, h, 0.0, c.lambda_t, dx, err);
-        const double mu_s = (e1.mu_s + e2.mu_s) / 2.0f;
-        double lambda_n = c.lambda_n;
-        double lambda_t = c.lambda_t + dl;
+        const S mu_s = (e1.mu_s + e2.mu_s) / 2.0f;
+        S lambda_n = c.lambda_n;
+        S lambda_t = c.lambda_t + dl;
         if (lambda_t > mu_s * lambda_n) {
-            V3 p1_til = add(e1.px, q_rot(e1.pq, c.r1));
-            V3 p2_til = add(e2.px, q_rot(e2.pq, c.r2));
-            V3 dp = sub(sub(p1, p1_til), sub(p2, p2_til));
-            V3 dp_t = sub(dp, scl(dot(dp, c.n), c.n));
+            V3T<S> p1_til = add(e1.px, q_rot(e1.pq, c.r1));
+            V3T<S> p2_til = add(e2.px, q_rot(e2.pq, c.r2));
+            V3T<S> dp = sub(sub(p1, p1_til), sub(p2, p2_til));
+            V3T<S> dp_t = sub(dp, scl(dot(dp, c.n), c.n));
             pos_apply(e1, e2, p, dl, dp_t);
             c.lambda_t += dl;
         }
@@ -823,27 +832,28 @@ RP_HD void solve_external(const ExtC& c, double* lam, Body& e1, Body& e2, double
 }
 
 // Velocity solve of one collision constraint (pbd.cpp:646-711)
-RP_HD void velocity_solve(const Contact& c, Body& e1, Body& e2, double h) {
-    V3 n = c.n;
-    double lambda_n = c.lambda_n;
-    PosPrep p;
+template <class S>
+RP_HD void velocity_solve(const ContactT<S>& c, BodyT<S>& e1, BodyT<S>& e2, double h) {
+    V3T<S> n = c.n;
+    S lambda_n = c.lambda_n;
+    PosPrepT<S> p;
     pos_prep(e1, e2, c.r1, c.r2, p);
-    V3 v = sub(add(e1.v, cross(e1.w, p.r1)), add(e2.v, cross(e2.w, p.r2)));
-    double vn = dot(n, v);
-    V3 vt = sub(v, scl(vn, n));
-    V3 dv = v3(0.0, 0.0, 0.0);
-    const double mu_d = (e1.mu_d + e2.mu_d) / 2.0f;
-    double fn = lambda_n / h;
-    double fact = RP_MIN(mu_d * fabs(fn), len(vt));
+    V3T<S> v = sub(add(e1.v, cross(e1.w, p.r1)), add(e2.v, cross(e2.w, p.r2)));
+    S vn = dot(n, v);
+    V3T<S> vt = sub(v, scl(vn, n));
+    V3T<S> dv = v3<S>(S(0.0), S(0.0), S(0.0));
+    const S mu_d = (e1.mu_d + e2.mu_d) / 2.0f;
+    S fn = lambda_n / h;
+    S fact = RP_MIN(mu_d * fabs(fn), len(vt));
     dv = add(dv, scl(-fact, nrm(vt)));
-    V3 vtil = sub(add(e1.pv, cross(e1.pw, p.r1)), add(e2.pv, cross(e2.pw, p.r2)));
-    double vn_til = dot(n, vtil);
-    double e = e1.e * e2.e;
-    fact = -vn + RP_MIN(-e * vn_til, 0.0);
+    V3T<S> vtil = sub(add(e1.pv, cross(e1.pw, p.r1)), add(e2.pv, cross(e2.pw, p.r2)));
+    S vn_til = dot(n, vtil);
+    S e = e1.e * e2.e;
+    fact = -vn + RP_MIN(-e * vn_til, S(0.0));
     dv = add(dv, scl(fact, n));
-    double w1 = e1.inv_mass + dot(cross(p.r1, n), m3_mulv(p.I1, cross(p.r1, n)));
-    double w2 = e2.inv_mass + dot(cross(p.r2, n), m3_mulv(p.I2, cross(p.r2, n)));
-    V3 pimp = scl(1.0 / (w1 + w2), dv);
+    S w1 = e1.inv_mass + dot(cross(p.r1, n), m3_mulv(p.I1, cross(p.r1, n)));
+    S w2 = e2.inv_mass + dot(cross(p.r2, n), m3_mulv(p.I2, cross(p.r2, n)));
+    V3T<S> pimp = scl(1.0 / (w1 + w2), dv);
     if (!e1.fixed) {
         e1.v = add(e1.v, scl(e1.inv_mass, pimp));
         e1.w = add(e1.w, m3_mulv(p.I1, cross(p.r1, pimp)));
@@ -855,17 +865,18 @@ RP_HD void velocity_solve(const Contact& c, Body& e1, Body& e2, double h) {
 }
 
 // Integration of one body (pbd.cpp:537-577)
-RP_HD void integrate(Body& e, double h) {
+template <class S>
+RP_HD void integrate(BodyT<S>& e, double h) {
     e.px = e.x;
     e.pq = e.q;
     if (e.fixed || !e.active) return;
     e.v = add(e.v, scl(h * e.inv_mass, e.F));
     e.x = add(e.x, scl(h, e.v));
-    M3 Iinv = world_tensor(e.q, e.Iinv);
-    M3 I = world_tensor(e.q, e.I);
+    M3T<S> Iinv = world_tensor(e.q, e.Iinv);
+    M3T<S> I = world_tensor(e.q, e.I);
     e.w = add(e.w, scl(h, m3_mulv(Iinv, sub(e.T, cross(e.w, m3_mulv(I, e.w))))));
-    Q4 aux = {e.w.x, e.w.y, e.w.z, 0.0};
-    Q4 q = q_mul(aux, e.q);
+    Q4T<S> aux = {e.w.x, e.w.y, e.w.z, S(0.0)};
+    Q4T<S> q = q_mul(aux, e.q);
     e.q.x = e.q.x + h * 0.5 * q.x;
     e.q.y = e.q.y + h * 0.5 * q.y;
     e.q.z = e.q.z + h * 0.5 * q.z;
@@ -874,14 +885,15 @@ RP_HD void integrate(Body& e, double h) {
 }
 
 // Velocity update of one body (pbd.cpp:623-643)
-RP_HD void velocity_update(Body& e, double h) {
+template <class S>
+RP_HD void velocity_update(BodyT<S>& e, double h) {
     if (e.fixed || !e.active) return;
     e.pv = e.v;
     e.pw = e.w;
     e.v = scl(1.0 / h, sub(e.x, e.px));
-    Q4 dq = q_mul(e.q, q_inv(e.pq));
-    if (dq.w >= 0.0) e.w = scl(2.0 / h, v3(dq.x, dq.y, dq.z));
-    else e.w = scl(-2.0 / h, v3(dq.x, dq.y, dq.z));
+    Q4T<S> dq = q_mul(e.q, q_inv(e.pq));
+    if (dq.w >= 0.0) e.w = scl(2.0 / h, v3<S>(dq.x, dq.y, dq.z));
+    else e.w = scl(-2.0 / h, v3<S>(dq.x, dq.y, dq.z));
 }
 
 // clipping_contact_to_collision_constraint (pbd.cpp:408-424)

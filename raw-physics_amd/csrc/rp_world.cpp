@@ -597,7 +597,27 @@ struct rp_world {
     uint64_t sched_gen = 1, graph_gen = 0;  // schedule changes invalidate the captured graph
     // bounded queue: at most two chunks of frames in flight on the stream
     hipEvent_t chunk_done[2] = {nullptr, nullptr};
+    // differentiable runs: per-frame body checkpoints of the last recorded
+    // forward pass and the buffers of the backward sweep
+    Body* ckpt = nullptr;
+    uint32_t ck_frames = 0;
+    FrameParams ck_fp;
+    int32_t* scene_dyn = nullptr;
+    Recorder rec = {nullptr, nullptr, nullptr, 0};
+    double* abar = nullptr;
+    double* gparam = nullptr;
 };
+
+static void adjoint_free(rp_world* w) {
+    void* ps[] = {w->ckpt, w->scene_dyn, w->rec.body, w->rec.ncontact, w->rec.contacts, w->abar, w->gparam};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+    w->ckpt = nullptr;
+    w->scene_dyn = nullptr;
+    w->rec = {nullptr, nullptr, nullptr, 0};
+    w->abar = w->gparam = nullptr;
+    w->ck_frames = 0;
+}
 
 // Frames per chunk of the bounded launch queue.
 constexpr uint32_t kFramesPerChunk = 32;
@@ -651,6 +671,7 @@ static void world_free(rp_world* w) {
     if (!w) return;
     if (w->stream) (void)hipStreamSynchronize(w->stream);
     if (w->frame_exec) (void)hipGraphExecDestroy(w->frame_exec);
+    adjoint_free(w);
     for (hipEvent_t e : w->chunk_done)
         if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : w->timer.pool) (void)hipEventDestroy(e);
@@ -916,6 +937,126 @@ uint32_t rp_fp64_selftest(const double* a, const double* b, double* out, uint64_
     if (db) (void)hipFree(db);
     if (dout) (void)hipFree(dout);
     return rc;
+}
+
+uint64_t rp_world_adjoint_mismatches(rp_world* w) { return read_stat(w, STAT_ADJ_MISMATCH); }
+
+uint32_t rp_world_set_body_params(rp_world* w, const double* mass, const double* mu_s, const double* mu_d) {
+    if (!download_bodies(w)) return RP_ERR_RUNTIME;
+    uint32_t err = 0;
+    for (size_t b = 0; b < w->host_bodies.size(); ++b) {
+        Body& e = w->host_bodies[b];
+        Body& h = w->hw.body[b];
+        if (mu_s) e.mu_s = h.mu_s = mu_s[b];
+        if (mu_d) e.mu_d = h.mu_d = mu_d[b];
+        if (mass && !e.fixed) {
+            // as at creation (make_body): 1/m, the hull's inertia and its inverse
+            const HostHull& hull = w->hw.hulls[w->hw.body_hull[b]];
+            e.inv_mass = h.inv_mass = 1.0 / mass[b];
+            e.I = h.I = hull_inertia(hull, mass[b]);
+            if (!m3_inverse(e.I, e.Iinv)) err |= ERR_HULL;
+            h.Iinv = e.Iinv;
+        }
+    }
+    if (!upload_bodies(w)) return RP_ERR_RUNTIME;
+    return err;
+}
+
+uint32_t rp_world_adjoint_forward(rp_world* w, double dt, uint32_t frames, uint32_t substeps, uint32_t iters,
+                                  double gravity) {
+    adjoint_free(w);
+    if (dt <= 0.0 || substeps == 0 || frames == 0) {
+        set_msg("adjoint_forward: dt, substeps and frames must be positive");
+        return RP_ERR_SELECT;
+    }
+    // scope: small scenes with one dynamic body and no external constraints
+    const DevWorld& d = w->im.d;
+    std::vector<int32_t> dyn(d.ns, -1);
+    for (int s = 0; s < d.ns; ++s) {
+        int nd = 0;
+        for (int b = w->hw.scene_body_off[s]; b < w->hw.scene_body_off[s + 1]; ++b)
+            if (!w->hw.body[b].fixed) { dyn[s] = b; ++nd; }
+        bool big = false;
+        for (const BigScene& bs : w->im.big) big |= bs.s == s;
+        if (nd > 1 || big || w->hw.scene_ext_off[s + 1] != w->hw.scene_ext_off[s]) {
+            set_msg("adjoint: a scene has more than one dynamic body, external constraints, or is a large scene");
+            return RP_ERR_ADJOINT_SCOPE;
+        }
+    }
+    size_t nb = (size_t)d.nb;
+    if (hipMalloc(&w->ckpt, (frames + 1) * nb * sizeof(Body)) != hipSuccess ||
+        hipMalloc(&w->scene_dyn, std::max<size_t>(1, dyn.size()) * 4) != hipSuccess ||
+        hipMemcpy(w->scene_dyn, dyn.data(), dyn.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
+        set_msg("adjoint_forward: device allocation failed");
+        adjoint_free(w);
+        return RP_ERR_RUNTIME;
+    }
+    FrameParams fp;
+    fp.dt = dt;
+    fp.h = dt / substeps;
+    fp.gravity = gravity;
+    fp.substeps = substeps;
+    fp.iters = iters;
+    fp.collisions = true;
+    for (uint32_t f = 0; f < frames; ++f) {
+        if (hipMemcpyAsync(w->ckpt + f * nb, d.body, nb * sizeof(Body), hipMemcpyDeviceToDevice, w->stream) !=
+            hipSuccess)
+            return RP_ERR_RUNTIME;
+        uint32_t e = world_run(w, fp, 1, false);
+        if (e) return e;
+    }
+    if (hipMemcpyAsync(w->ckpt + frames * nb, d.body, nb * sizeof(Body), hipMemcpyDeviceToDevice, w->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(w->stream) != hipSuccess)
+        return RP_ERR_RUNTIME;
+    w->ck_frames = frames;
+    w->ck_fp = fp;
+    return 0;
+}
+
+uint32_t rp_world_adjoint_backward(rp_world* w, const double* dl_dstate, double* grad_state, double* grad_param) {
+    if (!w->ckpt || w->ck_frames == 0) {
+        set_msg("adjoint_backward: no recorded forward pass");
+        return RP_ERR_SELECT;
+    }
+    const DevWorld& d = w->im.d;
+    size_t nb = (size_t)d.nb;
+    const FrameParams& fp = w->ck_fp;
+    if (!w->rec.body) {
+        w->rec.pair_cap = std::max<int64_t>(1, w->im.pair_capacity);
+        if (hipMalloc(&w->rec.body, fp.substeps * nb * sizeof(Body)) != hipSuccess ||
+            hipMalloc(&w->rec.ncontact, fp.substeps * w->rec.pair_cap * sizeof(int32_t)) != hipSuccess ||
+            hipMalloc(&w->rec.contacts, fp.substeps * w->rec.pair_cap * MAXC * sizeof(Contact)) != hipSuccess ||
+            hipMalloc(&w->abar, nb * 13 * sizeof(double)) != hipSuccess ||
+            hipMalloc(&w->gparam, nb * 3 * sizeof(double)) != hipSuccess) {
+            set_msg("adjoint_backward: device allocation failed");
+            return RP_ERR_RUNTIME;
+        }
+    }
+    if (hipMemcpy(w->abar, dl_dstate, nb * 13 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(w->gparam, 0, nb * 3 * sizeof(double)) != hipSuccess)
+        return RP_ERR_RUNTIME;
+    for (int f = (int)w->ck_frames - 1; f >= 0; --f) {
+        hipError_t e = hipMemcpyAsync(d.body, w->ckpt + f * nb, nb * sizeof(Body), hipMemcpyDeviceToDevice, w->stream);
+        if (e == hipSuccess)
+            e = launch_frame(d, fp, w->stream, nullptr, w->im.big.data(), (int)w->im.big.size(), &w->rec);
+        if (e == hipSuccess)
+            e = launch_adjoint_frame(d, w->rec, w->scene_dyn, w->ckpt + (f + 1) * nb, w->abar, w->gparam, fp,
+                                     w->stream);
+        if (e != hipSuccess) {
+            set_msg(std::string("adjoint_backward: ") + hipGetErrorString(e));
+            return RP_ERR_RUNTIME;
+        }
+    }
+    // the world continues from the end of the recorded run
+    if (hipMemcpyAsync(d.body, w->ckpt + w->ck_frames * nb, nb * sizeof(Body), hipMemcpyDeviceToDevice, w->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(w->stream) != hipSuccess)
+        return RP_ERR_RUNTIME;
+    if ((grad_state && hipMemcpy(grad_state, w->abar, nb * 13 * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) ||
+        (grad_param && hipMemcpy(grad_param, w->gparam, nb * 3 * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess))
+        return RP_ERR_RUNTIME;
+    return 0;
 }
 
 int32_t rp_world_set_narrow_group(rp_world* w, int32_t lanes_per_pair) {

@@ -394,6 +394,42 @@ def batched_uniform(num_scenes: int, first_seed: int = 0, **kw) -> World:
     return w
 
 
+def ramp_scene(w: World, seed: int) -> np.ndarray:
+    """One differentiable-workload scene: a static box cube.obj x (10, 0.5, 4)
+    tilted about z by U(10, 35) degrees at the origin, and a cube of
+    half-extent 1 with mass U(0.5, 2) whose centre starts 1.5 + sqrt(3) above
+    the centre of the ramp's top face (clear of it in any orientation), with
+    v ~ N(0, 2)^3, w ~ N(0, 3)^3 and a uniformly random orientation; friction
+    0.6/0.5 for both bodies, restitution 0. Returns the loss target: the foot
+    of the ramp plus a U([-2,2] x [0,1] x [-2,2]) offset."""
+    rng = Rng(0x5EED0000 + seed)
+    cube = w.shape(cube_corners())
+    angle = rng.uniform(10.0, 35.0)
+    qr = quaternion_new((0.0, 0.0, 1.0), angle)
+    w.add_body(cube, (10.0, 0.5, 4.0), (0.0, 0.0, 0.0), qr, fixed=True, material=(0.6, 0.5, 0.0))
+    top = _qrot(qr, (0.0, 0.5, 0.0))
+    normal = _qrot(qr, (0.0, 1.0, 0.0))
+    lift = 1.5 + math.sqrt(3.0)
+    x = tuple(top[i] + lift * normal[i] for i in range(3))
+    mass = rng.uniform(0.5, 2.0)
+    v = (rng.normal(2.0), rng.normal(2.0), rng.normal(2.0))
+    om = (rng.normal(3.0), rng.normal(3.0), rng.normal(3.0))
+    w.add_body(cube, (1.0, 1.0, 1.0), x, rng.quaternion(), mass=mass, material=(0.6, 0.5, 0.0), linear_velocity=v,
+               angular_velocity=om)
+    w.end_scene()
+    foot = _qrot(qr, (-10.0, 0.5, 0.0))
+    return np.array([foot[0] + rng.uniform(-2.0, 2.0), foot[1] + rng.uniform(0.0, 1.0),
+                     foot[2] + rng.uniform(-2.0, 2.0)])
+
+
+def ramps(num_scenes: int, first_seed: int = 0):
+    """The differentiable workload: `num_scenes` ramp scenes; returns the
+    world and the (num_scenes, 3) loss targets. Body 2s+1 is scene s's cube."""
+    w = World(name=f"ramps_{num_scenes}")
+    targets = np.stack([ramp_scene(w, first_seed + s) for s in range(num_scenes)])
+    return w, targets
+
+
 LARGE_SCALES = 64  # scale levels of the large scene (bodies of one level share a hull)
 
 

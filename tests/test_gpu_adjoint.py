@@ -1,0 +1,83 @@
+"""The device adjoint (rp_adjoint.hip) against the oracle's dual-number build.
+
+Scenes: a cube thrown onto a static tilted box (the differentiable workload).
+The forward pass must match the oracle bit for bit (contact scenes), the
+backward pass must recompute every substep bit for bit (no mismatches), and
+the gradients of L = sum_s |x_cube(s) - target_s|^2 with respect to initial
+velocity, angular velocity, mass and friction must equal the frozen-contact
+forward-mode derivatives of the oracle (the same map) to rounding.
+"""
+import numpy as np
+import pytest
+
+from _libs import SimWorld, pkg, scenes, state_bits_equal
+from test_oracle_dual import PARAMS, _libs, dual_gradients
+
+pytestmark = pytest.mark.gpu
+
+# dual slot -> column of the device gradient: state (x, q, v, w) or parameter
+DEVICE_COLUMN = {"vx": ("s", 7), "vy": ("s", 8), "vz": ("s", 9), "wx": ("s", 10), "wz": ("s", 12),
+                 "mass": ("p", 0), "mu_s": ("p", 1), "mu_d": ("p", 2)}
+
+
+def _device_gradients(world, targets, frames):
+    gw = pkg.World(world)
+    assert gw.adjoint_forward(frames) == 0
+    st = gw.state()
+    dl = np.zeros((world.num_bodies, 13))
+    for s in range(len(targets)):
+        dl[2 * s + 1, 0:3] = 2.0 * (st["position"][2 * s + 1] - targets[s])
+    gs, gp = gw.adjoint_backward(dl)
+    assert gw.adjoint_mismatches() == 0, "backward recomputation differs from the forward pass"
+    g = np.zeros((len(targets), len(PARAMS)))
+    for slot, (name, _, _) in enumerate(PARAMS):
+        kind, col = DEVICE_COLUMN[name]
+        for s in range(len(targets)):
+            g[s, slot] = gs[2 * s + 1, col] if kind == "s" else gp[2 * s + 1, col]
+    return g, st, gw
+
+
+@pytest.mark.parametrize("frames", [30, 120])
+def test_adjoint_matches_frozen_dual_oracle(frames):
+    _, dual = _libs()
+    world, targets = scenes.ramps(12, first_seed=100)
+    g_dev, st_dev, gw = _device_gradients(world, targets, frames)
+    g_ref, st_ref = dual_gradients(dual, world, targets, frames, freeze=True)
+    assert not state_bits_equal(st_dev, st_ref), "forward pass differs from the oracle"
+    scale = np.maximum(np.abs(g_ref), 1e-6 * np.abs(g_ref).max(axis=1, keepdims=True) + 1e-12)
+    rel = np.abs(g_dev - g_ref) / scale
+    print(f"{frames} frames: max relative difference device adjoint vs dual oracle {rel.max():.3e}")
+    assert rel.max() <= 1e-7, np.argwhere(rel > 1e-7)
+    for s in range(world.num_scenes):
+        assert gw.errors()[s] == 0
+
+
+def test_autograd_function_matches_direct_adjoint():
+    import torch
+    from raw_physics_amd.autograd import ThrowRun, ThrowSetup
+
+    world, targets = scenes.ramps(6, first_seed=200)
+    frames = 40
+    run = ThrowSetup(world, frames)
+    dyn = run.dynamic
+    v0 = torch.tensor(np.asarray(world.body_linear_velocity)[dyn], dtype=torch.float64, requires_grad=True)
+    w0 = torch.tensor(np.asarray(world.body_angular_velocity)[dyn], dtype=torch.float64, requires_grad=True)
+    mass = torch.tensor(np.asarray(world.body_mass)[dyn], dtype=torch.float64, requires_grad=True)
+    mat = np.asarray(world.body_material)[dyn]
+    mu_s = torch.tensor(mat[:, 0], dtype=torch.float64, requires_grad=True)
+    mu_d = torch.tensor(mat[:, 1], dtype=torch.float64, requires_grad=True)
+    x = ThrowRun.apply(v0, w0, mass, mu_s, mu_d, run)
+    loss = ((x - torch.from_numpy(targets)) ** 2).sum()
+    loss.backward()
+    g_dev, _, _ = _device_gradients(world, targets, frames)
+    cols = {name: k for k, (name, _, _) in enumerate(PARAMS)}
+    assert np.allclose(v0.grad.numpy()[:, 0], g_dev[:, cols["vx"]], rtol=1e-12, atol=0)
+    assert np.allclose(w0.grad.numpy()[:, 2], g_dev[:, cols["wz"]], rtol=1e-12, atol=0)
+    assert np.allclose(mass.grad.numpy(), g_dev[:, cols["mass"]], rtol=1e-12, atol=0)
+    assert np.allclose(mu_d.grad.numpy(), g_dev[:, cols["mu_d"]], rtol=1e-12, atol=0)
+
+
+def test_adjoint_scope_is_enforced():
+    world = scenes.batched_uniform(1)
+    gw = pkg.World(world)
+    assert gw.adjoint_forward(2) != 0  # 16 dynamic bodies in one scene
