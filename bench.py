@@ -79,7 +79,7 @@ def cpu_baseline(scenes_mod, frames: int, budget_s: float):
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (and matrix) peak
 
 
-def op_counts(scenes_mod, n_scenes: int, frames: int):
+def op_counts(scenes_mod, n_scenes: int, frames: int, start: int = 0):
     """Algorithmic FP64 work per phase on a sample of the workload, counted by
     the oracle's op-counting build (oracle/libopcount.so: every add, subtract,
     multiply, divide and square root the restatement executes)."""
@@ -93,6 +93,8 @@ def op_counts(scenes_mod, n_scenes: int, frames: int):
     lib.orc_opcount_get.argtypes = [ctypes.c_void_p]
     world = scenes_mod.batched_uniform(n_scenes, first_seed=0)
     sim = SimWorld("orc", world, lib=lib)
+    if start:
+        sim.step(start)  # the counted frames start where the timed window starts
     lib.orc_opcount_reset()
     pairs = contacts = 0
     for _ in range(frames):
@@ -108,6 +110,77 @@ def op_counts(scenes_mod, n_scenes: int, frames: int):
     d["solve"] = d["position"] + d["velocity_update"] + d["velocity_solve"]
     return {"ops": d, "scenes": n_scenes, "frames": frames, "narrow_pairs": int(pairs), "contacts": int(contacts),
             "body_substeps": world.dynamic_bodies() * 20 * frames}
+
+
+TRAFFIC_FILE = ROOT / "profiles" / "traffic_window_f300_batched_uniform_4096x16.json"
+
+
+def committed_traffic(kernel_class: str, lanes: int):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 PMC
+    summary (FETCH_SIZE and WRITE_SIZE passes, scaled by the copy-kernel
+    calibration), when it covers this kernel and schedule; else None."""
+    if not TRAFFIC_FILE.exists():
+        return None
+    d = json.loads(TRAFFIC_FILE.read_text())
+    k = d.get("kernels", {}).get(kernel_class)
+    if not k or k.get("lanes") != lanes:
+        return None
+    return {"bytes_per_launch": k["bytes_per_launch"], "source": f"{TRAFFIC_FILE.relative_to(ROOT)}: {d['window']}"}
+
+
+def adjoint_run(pkg, sc, n_scenes: int, frames: int):
+    """Differentiable workload: n_scenes cubes thrown onto tilted static boxes,
+    `frames` frames; the checkpointed forward pass, then the backward pass of
+    L = sum |x_final - target|^2. Adjoint steps/s = scenes x frames x
+    substeps reversed per second of the backward pass (recomputation included)."""
+    world, targets = sc.ramps(n_scenes)
+    gw = pkg.World(world)
+    init = gw.state()
+    gw.adjoint_forward(2)  # first launches
+    gw.set_state(init)
+    gw.synchronize()
+    t0 = time.perf_counter()
+    err = gw.adjoint_forward(frames)
+    gw.synchronize()
+    t_fwd = time.perf_counter() - t0
+    st = gw.state()
+    dl = np.zeros((world.num_bodies, 13))
+    dyn = np.arange(1, world.num_bodies, 2)
+    dl[dyn, 0:3] = 2.0 * (st["position"][dyn] - targets)
+    t0 = time.perf_counter()
+    gs, gp = gw.adjoint_backward(dl)
+    t_bwd = time.perf_counter() - t0
+    out = {"metric": "adjoint steps/s", "unit": "scene-substeps reversed/s",
+           "value": n_scenes * frames * 20 / t_bwd,
+           "config": {"workload": f"differentiable: {n_scenes} scenes x (cube thrown onto a tilted static box), "
+                                  f"{frames} frames, dt=1/60, 20 substeps, contact set frozen",
+                      "scenes": n_scenes, "frames": frames},
+           "forward_checkpointed_s": t_fwd, "backward_s": t_bwd, "forward_error": int(err),
+           "recompute_mismatches": gw.adjoint_mismatches(),
+           "grad_norm_v0": float(np.linalg.norm(gs[dyn, 7:10])), "grad_norm_mass": float(np.linalg.norm(gp[dyn, 0]))}
+    gw.close()
+    log("adjoint: " + json.dumps(out))
+    return out
+
+
+def large_run(pkg, sc, n: int, frames: int):
+    """One large scene (grid broad phase, coloured solve): body-substeps/s."""
+    t0 = time.perf_counter()
+    world = sc.large_scene(n)
+    t_build = time.perf_counter() - t0
+    gw = pkg.World(world)
+    gw.step(2)
+    gw.synchronize()
+    t0 = time.perf_counter()
+    gw.step(frames, sync=False)
+    gw.synchronize()
+    el = time.perf_counter() - t0
+    out = {"bodies": n, "frames_timed": frames, "after_frames": 2,
+           "value": n * 20 * frames / el, "unit": "body-substeps/s", "ms_per_frame": 1e3 * el / frames,
+           "scene_build_s": t_build, "error_word": int(gw.errors()[0])}
+    gw.close()
+    log("large scene: " + json.dumps(out))
+    return out
 
 
 def copy_calibration(nbytes: int):
@@ -138,6 +211,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel timing run")
     ap.add_argument("--op-scenes", type=int, default=8, help="scenes in the op-count sample")
+    ap.add_argument("--op-start", type=int, default=0,
+                    help="frame the op-count sample starts at (the frame a --load-state window starts at)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tune-from", type=int, default=90, help="frame the autotuner's trials start from")
     ap.add_argument("--tune-frames", type=int, default=6)
@@ -145,6 +220,11 @@ def main():
     ap.add_argument("--manifold-lanes", type=int, default=0, help="fixed manifold schedule (lanes per pair); skips the autotuner")
     ap.add_argument("--solve-lanes", type=int, default=0, help="fixed solve schedule (lanes per scene); skips the autotuner")
     ap.add_argument("--eager", action="store_true", help="launch each kernel eagerly instead of replaying frame graphs")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurement")
+    ap.add_argument("--adjoint-scenes", type=int, default=2048)
+    ap.add_argument("--adjoint-frames", type=int, default=120)
+    ap.add_argument("--large-bodies", type=int, default=0, help="also time one large scene of this many bodies")
+    ap.add_argument("--large-frames", type=int, default=30)
     ap.add_argument("--save-state", metavar="NPZ", help="after the timed frames, save the batch's body state")
     ap.add_argument("--load-state", metavar="NPZ",
                     help="start the timed frames from a saved body state (a window deep in the run, for profiles)")
@@ -228,6 +308,9 @@ def main():
         log("kernel times (ms, launches): " + json.dumps(ktimes))
     gw.close()
 
+    adjoint = None if args.no_adjoint else adjoint_run(pkg, sc, args.adjoint_scenes, args.adjoint_frames)
+    large = None if not args.large_bodies else large_run(pkg, sc, args.large_bodies, args.large_frames)
+
     plain = None
     if not args.no_plain:
         ps = sc.batched_uniform(args.plain_scenes)
@@ -257,7 +340,7 @@ def main():
     roofline = None
     kernels = None
     if ktimes is not None:
-        oc = op_counts(sc, min(args.op_scenes, args.scenes), args.steps)
+        oc = op_counts(sc, min(args.op_scenes, args.scenes), args.steps, args.op_start)
         if oc is not None:
             scale = args.scenes / oc["scenes"]
             kernels = {}
@@ -276,10 +359,13 @@ def main():
             unit_ops = {"narrow": (oc["ops"]["narrow"], oc["narrow_pairs"], "live pair"),
                         "manifold": (oc["ops"]["manifold"], oc["narrow_pairs"], "live pair"),
                         "solve": (oc["ops"]["solve"], oc["body_substeps"], "body-substep")}[dom]
+            traffic = committed_traffic(dom, {"narrow": narrow_group, "manifold": manifold_group,
+                                              "solve": solve_lanes}[dom])
             roofline = {"bound": "fp64-valu", "kernel": names[dom],
                         "achieved": k["tflops"], "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                         "frac": k["tflops"] / FP64_PEAK_TFLOPS if k["tflops"] else None,
-                        "traffic": None,
+                        "traffic": traffic["bytes_per_launch"] if traffic else None,
+                        "traffic_source": traffic["source"] if traffic else None,
                         "flops_per_launch": k["algorithmic_flops"] / max(k["launches"], 1),
                         "avg_launch_us": k["avg_us"],
                         "op_sample": f"{oc['scenes']} scenes x {oc['frames']} frames counted by the oracle's "
@@ -307,6 +393,8 @@ def main():
                    "solve_lanes_per_scene": solve_lanes, "autotune_ms": tune},
         "plain": {"value": plain, "unit": "body-substeps/s",
                   "ratio_batched_over_plain": (value / plain) if plain else None},
+        "adjoint": adjoint,
+        "large_scene": large,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,

@@ -135,11 +135,50 @@ RP_HD int support_index(const HullView& h, V3 d, uint32_t& err) {
     if (sel < 0) { err |= ERR_SUPPORT_NAN; sel = 0; }
     return sel;
 }
+// Support indices of two hulls in directions d1 and d2 at once. For G = 1
+// the two scans advance together, eight vertices of each hull per batch, so
+// their memory latencies overlap; each scan keeps its own first-maximum
+// order. For G > 1 the two group scans run one after the other.
+template <int G = 1>
+RP_HD void support_index2(const HullView& a, V3 d1, const HullView& b, V3 d2, int& i1, int& i2, uint32_t& err) {
+    if constexpr (G == 1) {
+        int s1 = -1, s2 = -1;
+        double m1 = -DBL_MAX, m2 = -DBL_MAX;
+        int n = a.nv > b.nv ? a.nv : b.nv;
+        for (int i = 0; i < n; i += 8) {
+            V3 va[8], vb[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                if (i + k < a.nv) va[k] = a.tv[i + k];
+                if (i + k < b.nv) vb[k] = b.tv[i + k];
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                if (i + k < a.nv) {
+                    double dd = dot(va[k], d1);
+                    if (dd > m1) { s1 = i + k; m1 = dd; }
+                }
+                if (i + k < b.nv) {
+                    double dd = dot(vb[k], d2);
+                    if (dd > m2) { s2 = i + k; m2 = dd; }
+                }
+            }
+        }
+        if (s1 < 0) { err |= ERR_SUPPORT_NAN; s1 = 0; }
+        if (s2 < 0) { err |= ERR_SUPPORT_NAN; s2 = 0; }
+        i1 = s1;
+        i2 = s2;
+    } else {
+        i1 = support_index<G>(a, d1, err);
+        i2 = support_index<G>(b, d2, err);
+    }
+}
+
 template <int G = 1>
 RP_HD V3 support_md(const HullView& a, const HullView& b, V3 d, uint32_t& err) {
-    V3 s1 = a.tv[support_index<G>(a, d, err)];
-    V3 s2 = b.tv[support_index<G>(b, scl(-1.0, d), err)];
-    return sub(s1, s2);
+    int i1, i2;
+    support_index2<G>(a, d, b, scl(-1.0, d), i1, i2, err);
+    return sub(a.tv[i1], b.tv[i2]);
 }
 
 // ---------------------------------------------------------------------------
@@ -502,8 +541,8 @@ template <int G = 1>
 RP_HD int manifold(const HullView& h1, const HullView& h2, V3 normal, ContactPt* out, uint32_t& err) {
     const double EPSILON = 0.0001;
     V3 inormal = inv3(normal);
-    int s1 = support_index<G>(h1, normal, err);
-    int s2 = support_index<G>(h2, inormal, err);
+    int s1, s2;
+    support_index2<G>(h1, normal, h2, inormal, s1, s2, err);
     int f1 = best_face(s1, h1, normal, err);
     int f2 = best_face(s2, h2, inormal, err);
     // get_edge_with_most_fitting_normal (clipping.cpp:154-200)
