@@ -189,6 +189,14 @@ typedef struct {
      * sequential constraint order (results then differ from the reference
      * within a tolerance, and do not depend on thread timing). */
     uint32_t large_scene_bodies;
+    /* Spheres and compound bodies (NULL: every shape is a hull and every body
+     * has the one collider body_shape[g]). A shape with radius > 0 is a sphere
+     * collider of that radius (collider_sphere_create, collider.cpp:12-18;
+     * body_scale does not apply). Body g's colliders, in order, are
+     * collider_shape[body_collider_offset[g] .. body_collider_offset[g+1]). */
+    const float* shape_sphere_radius;     /* [num_shapes] or NULL */
+    const uint32_t* body_collider_offset; /* [nb+1] or NULL */
+    const uint32_t* collider_shape;       /* [colliders] */
 } rp_world_desc;
 
 /* Per-body state as flat arrays; NULL members are skipped. */

@@ -42,25 +42,38 @@ API void* ref_world_create(const orc_world_desc* d) {
         Entity** ents = array_new(Entity*);
         std::vector<eid> ids;
         for (uint32_t g = d->scene_body_offset[si]; g < d->scene_body_offset[si + 1]; ++g) {
-            uint32_t shp = d->body_shape[g];
-            uint32_t c0 = d->shape_corner_offset[shp], c1 = d->shape_corner_offset[shp + 1];
-            /* The corner stream obj_parse would return (src/render/obj.cpp:45-107). */
-            Vertex* verts = array_new(Vertex);
-            u32* idx = array_new(u32);
-            for (uint32_t c = c0; c < c1; ++c) {
-                Vertex v;
-                memset(&v, 0, sizeof(v));
-                v.position.x = d->shape_corners[3 * c];
-                v.position.y = d->shape_corners[3 * c + 1];
-                v.position.z = d->shape_corners[3 * c + 2];
-                array_push(verts, v);
-                u32 k = c - c0;
-                array_push(idx, k);
-            }
             vec3 scale = (vec3){d->body_scale[3 * g], d->body_scale[3 * g + 1], d->body_scale[3 * g + 2]};
-            Collider* cols = examples_util_create_single_convex_hull_collider_array(verts, idx, scale);
-            array_free(verts);
-            array_free(idx);
+            /* The body's colliders in order: spheres by collider_sphere_create,
+             * hulls from the corner stream obj_parse would return
+             * (src/render/obj.cpp:45-107) as examples_util does (examples_util.cpp:5-49). */
+            uint32_t k0 = d->body_collider_offset ? d->body_collider_offset[g] : 0;
+            uint32_t nc = d->body_collider_offset ? d->body_collider_offset[g + 1] - k0 : 1;
+            Collider* cols = array_new(Collider);
+            for (uint32_t ci = 0; ci < nc; ++ci) {
+                uint32_t shp = d->body_collider_offset ? d->collider_shape[k0 + ci] : d->body_shape[g];
+                if (d->shape_sphere_radius && d->shape_sphere_radius[shp] > 0.0f) {
+                    Collider sph = collider_sphere_create(d->shape_sphere_radius[shp]);
+                    array_push(cols, sph);
+                    continue;
+                }
+                uint32_t c0 = d->shape_corner_offset[shp], c1 = d->shape_corner_offset[shp + 1];
+                Vertex* verts = array_new(Vertex);
+                u32* idx = array_new(u32);
+                for (uint32_t c = c0; c < c1; ++c) {
+                    Vertex v;
+                    memset(&v, 0, sizeof(v));
+                    v.position.x = d->shape_corners[3 * c];
+                    v.position.y = d->shape_corners[3 * c + 1];
+                    v.position.z = d->shape_corners[3 * c + 2];
+                    array_push(verts, v);
+                    u32 k = c - c0;
+                    array_push(idx, k);
+                }
+                Collider hull = examples_util_create_convex_hull_collider(verts, idx, scale);
+                array_push(cols, hull);
+                array_free(verts);
+                array_free(idx);
+            }
             Mesh mesh;
             memset(&mesh, 0, sizeof(mesh));
             vec3 pos = (vec3){d->body_position[3 * g], d->body_position[3 * g + 1], d->body_position[3 * g + 2]};

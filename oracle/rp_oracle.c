@@ -256,6 +256,16 @@ typedef struct {
     int stamp; /* marked for transformation this substep */
 } hull;
 
+/* Collider (collider.h:27-44): a convex hull, or a sphere whose center
+ * follows the entity position (collider.cpp:431-433). */
+enum { COL_HULL = 0, COL_SPHERE = 1 };
+typedef struct collider {
+    int32_t type;
+    hull h;
+    float radius;  /* r32 (collider.h:29) */
+    v3 center;
+} collider;
+
 typedef struct {
     v3 x; qt q; v3 v; v3 w;      /* world_position, world_rotation, linear/angular velocity */
     v3 px; qt pq; v3 pv; v3 pw;  /* previous_* */
@@ -266,7 +276,9 @@ typedef struct {
     real radius;
     real mu_s, mu_d, e;
     v3 force, torque;            /* external force / torque of the current frame */
-    hull h;
+    struct collider* col;        /* the entity's colliders, in order */
+    uint32_t ncol;
+    int stamp;                   /* colliders to be updated this substep */
 } body;
 
 typedef struct {
@@ -537,9 +549,15 @@ static uint32_t support_index(const hull* h, v3 d, uint32_t* err) {
     if (sel == UINT32_MAX) { *err |= ORC_ERR_SUPPORT_NAN; sel = 0; }
     return sel;
 }
-static v3 support_md(const hull* a, const hull* b, v3 d, uint32_t* err) {
-    v3 s1 = a->tv[support_index(a, d, err)];
-    v3 s2 = b->tv[support_index(b, scl(-1.0, d), err)];
+/* support_point (support.cpp:19-31) */
+static v3 support_point(const collider* c, v3 d, uint32_t* err) {
+    if (c->type == COL_SPHERE) return add(c->center, scl((real)c->radius, nrm(d)));
+    return c->h.tv[support_index(&c->h, d, err)];
+}
+/* support_point_of_minkowski_difference (support.cpp:34-39) */
+static v3 support_md(const collider* a, const collider* b, v3 d, uint32_t* err) {
+    v3 s1 = support_point(a, d, err);
+    v3 s2 = support_point(b, scl(-1.0, d), err);
     return sub(s1, s2);
 }
 
@@ -661,7 +679,7 @@ static int do_simplex_4(simplex* s, v3* dir) {
 }
 
 /* gjk_collides (gjk.cpp:350-379) */
-static int gjk(const hull* h1, const hull* h2, simplex* out, uint32_t* err) {
+static int gjk(const collider* h1, const collider* h2, simplex* out, uint32_t* err) {
     simplex s;
     memset(&s, 0, sizeof(s));
     s.a = support_md(h1, h2, V(0.0, 0.0, 1.0), err);
@@ -729,7 +747,7 @@ static void add_edge(i2arr* edges, i2 e, const v3* poly) {
 static v3 centroid(v3 p1, v3 p2, v3 p3) { return scl(1.0 / 3.0, add(add(p2, p3), p1)); }
 
 /* epa (epa.cpp:118-237). Returns 1 on convergence. */
-static int epa(const hull* h1, const hull* h2, const simplex* s, v3* n_out, real* pen_out, uint32_t* err) {
+static int epa(const collider* h1, const collider* h2, const simplex* s, v3* n_out, real* pen_out, uint32_t* err) {
     v3arr poly = {0};
     i3arr faces = {0};
     v3arr normals = {0};
@@ -989,15 +1007,69 @@ static void manifold(const hull* h1, const hull* h2, v3 normal, contactarr* out,
     }
 }
 
-/* collider_get_contacts for two convex hulls (collider.cpp:523-558). */
-static void pair_contacts(const hull* h1, const hull* h2, contactarr* out, uint32_t* err, uint64_t* epa_fail) {
+/* clipping_get_contact_manifold (clipping.cpp:343-371): a sphere gives one
+ * contact at its support point along the normal. */
+static void contact_manifold(const collider* c1, const collider* c2, v3 normal, real penetration, contactarr* out,
+                             uint32_t* err) {
+    if (c1->type == COL_SPHERE) {
+        v3 p = support_point(c1, normal, err);
+        contact c;
+        c.p1 = p;
+        c.p2 = sub(p, scl(penetration, normal));
+        c.n = normal;
+        ARR_PUSH(*out, c);
+    } else if (c2->type == COL_SPHERE) {
+        v3 inormal = inv3(normal);
+        v3 p = support_point(c2, inormal, err);
+        contact c;
+        c.p1 = add(p, scl(penetration, normal));
+        c.p2 = p;
+        c.n = normal;
+        ARR_PUSH(*out, c);
+    } else {
+        manifold(&c1->h, &c2->h, normal, out, err);
+    }
+}
+
+/* collider_get_contacts (collider.cpp:523-558). Two spheres are resolved
+ * analytically with the reference's float arithmetic; otherwise GJK, EPA and
+ * the contact manifold. */
+static void collider_contacts(const collider* h1, const collider* h2, contactarr* out, uint32_t* err,
+                              uint64_t* epa_fail) {
+    if (h1->type == COL_SPHERE && h2->type == COL_SPHERE) {
+        v3 dv = sub(h1->center, h2->center);
+        float dsq = (float)RD(dot(dv, dv));
+        float mind = h1->radius + h2->radius;
+        if (dsq < (mind * mind)) {
+            v3 n = nrm(sub(h2->center, h1->center));
+            real pen = (real)(double)(mind - sqrtf(dsq));
+            ORC_PHASE(PH_MANIFOLD);
+            contact_manifold(h1, h2, n, pen, out, err);
+            ORC_PHASE(PH_NARROW);
+        }
+        return;
+    }
     simplex s;
     if (gjk(h1, h2, &s, err)) {
         v3 n; real pen;
         if (!epa(h1, h2, &s, &n, &pen, err)) { ++*epa_fail; return; }
         ORC_PHASE(PH_MANIFOLD);
-        manifold(h1, h2, n, out, err);
+        contact_manifold(h1, h2, n, pen, out, err);
         ORC_PHASE(PH_NARROW);
+    }
+}
+
+/* colliders_get_contacts (collider.cpp:560-572): every collider pair, in order. */
+static void pair_contacts(const body* e1, const body* e2, contactarr* out, uint32_t* err, uint64_t* epa_fail) {
+    for (uint32_t i = 0; i < e1->ncol; ++i)
+        for (uint32_t j = 0; j < e2->ncol; ++j) collider_contacts(&e1->col[i], &e2->col[j], out, err, epa_fail);
+}
+
+/* colliders_update (collider.cpp:409-445) */
+static void colliders_update(body* e) {
+    for (uint32_t i = 0; i < e->ncol; ++i) {
+        if (e->col[i].type == COL_SPHERE) e->col[i].center = e->x;
+        else hull_transform(&e->col[i].h, e->x, e->q);
     }
 }
 
@@ -1453,12 +1525,12 @@ static void simulate(orc_world* w, scene* sc, real dt, uint32_t substeps, uint32
             for (uint32_t k = 0; k < np; ++k) {  /* bodies of the pairs visited below, once each */
                 body *e1 = &sc->b[pairs[2 * k]], *e2 = &sc->b[pairs[2 * k + 1]];
                 if ((e1->fixed || !e1->active) && (e2->fixed || !e2->active)) continue;
-                e1->h.stamp = e2->h.stamp = 1;
+                e1->stamp = e2->stamp = 1;
             }
             for (uint32_t j = 0; j < sc->nb; ++j) {
-                if (!sc->b[j].h.stamp) continue;
-                sc->b[j].h.stamp = 0;
-                hull_transform(&sc->b[j].h, sc->b[j].x, sc->b[j].q);
+                if (!sc->b[j].stamp) continue;
+                sc->b[j].stamp = 0;
+                colliders_update(&sc->b[j]);
             }
             ORC_PHASE(PH_NARROW);
             for (uint32_t k = 0; k < np; ++k) {
@@ -1491,7 +1563,7 @@ static void simulate(orc_world* w, scene* sc, real dt, uint32_t substeps, uint32
                     continue;
                 }
                 cts.n = 0;
-                pair_contacts(&e1->h, &e2->h, &cts, &sc->err, &w->epa_failures);
+                pair_contacts(e1, e2, &cts, &sc->err, &w->epa_failures);
                 for (uint32_t l = 0; l < cts.n; ++l) {
                     constraint c = contact_to_constraint(e1, e2, i1, i2, &cts.d[l]);
                     ARR_PUSH(cons, c);
@@ -1563,19 +1635,34 @@ orc_world* orc_world_create(const orc_world_desc* d) {
         for (uint32_t k = 0; k < sc->nb; ++k) {
             uint32_t g = b0 + k;
             body* e = &sc->b[k];
-            uint32_t shp = d->body_shape[g];
-            uint32_t c0 = d->shape_corner_offset[shp], c1 = d->shape_corner_offset[shp + 1];
-            v3* corners = malloc(sizeof(v3) * ((c1 - c0) ? (c1 - c0) : 1));
-            v3 sc3 = ld3(d->body_scale, g);
-            /* examples_util.cpp:7-16: float -> double, then *= scale */
-            for (uint32_t c = c0; c < c1; ++c) {
-                v3 p = V((double)d->shape_corners[3 * c], (double)d->shape_corners[3 * c + 1],
-                         (double)d->shape_corners[3 * c + 2]);
-                p.x *= sc3.x; p.y *= sc3.y; p.z *= sc3.z;
-                corners[c - c0] = p;
+            /* colliders: the body's shape list, hull corners widened and scaled
+             * (examples_util.cpp:7-16), spheres of their shape's radius */
+            uint32_t k0 = d->body_collider_offset ? d->body_collider_offset[g] : 0;
+            uint32_t nc = d->body_collider_offset ? d->body_collider_offset[g + 1] - k0 : 1;
+            e->ncol = nc;
+            e->col = calloc(nc ? nc : 1, sizeof(collider));
+            for (uint32_t ci = 0; ci < nc; ++ci) {
+                uint32_t shp = d->body_collider_offset ? d->collider_shape[k0 + ci] : d->body_shape[g];
+                collider* col = &e->col[ci];
+                if (d->shape_sphere_radius && d->shape_sphere_radius[shp] > 0.0f) {
+                    col->type = COL_SPHERE;
+                    col->radius = d->shape_sphere_radius[shp];
+                    col->center = V(0.0, 0.0, 0.0);
+                    continue;
+                }
+                col->type = COL_HULL;
+                uint32_t c0 = d->shape_corner_offset[shp], c1 = d->shape_corner_offset[shp + 1];
+                v3* corners = malloc(sizeof(v3) * ((c1 - c0) ? (c1 - c0) : 1));
+                v3 sc3 = ld3(d->body_scale, g);
+                for (uint32_t c = c0; c < c1; ++c) {
+                    v3 p = V((double)d->shape_corners[3 * c], (double)d->shape_corners[3 * c + 1],
+                             (double)d->shape_corners[3 * c + 2]);
+                    p.x *= sc3.x; p.y *= sc3.y; p.z *= sc3.z;
+                    corners[c - c0] = p;
+                }
+                if (!hull_build(&col->h, corners, c1 - c0)) sc->err |= ORC_ERR_HULL;
+                free(corners);
             }
-            if (!hull_build(&e->h, corners, c1 - c0)) sc->err |= ORC_ERR_HULL;
-            free(corners);
             /* entity_create_ex (entity.cpp:24-65) */
             e->x = ld3(d->body_position, g);
             e->q = ld4(d->body_rotation, g);
@@ -1585,13 +1672,22 @@ orc_world* orc_world_create(const orc_world_desc* d) {
             e->pw = V(0.0, 0.0, 0.0);
             e->px = V(0.0, 0.0, 0.0);
             e->pq.x = e->pq.y = e->pq.z = e->pq.w = 0.0;
-            /* colliders_get_bounding_sphere_radius over one hull (collider.cpp:178-189, 510-521) */
-            real maxd = 0.0;
-            for (uint32_t i = 0; i < e->h.nv; ++i) {
-                real l = len(e->h.v[i]);
-                if (l > maxd) maxd = l;
+            /* colliders_get_bounding_sphere_radius (collider.cpp:178-189, 510-521) */
+            real maxr = -DBL_MAX;
+            for (uint32_t ci = 0; ci < e->ncol; ++ci) {
+                const collider* col = &e->col[ci];
+                real r = 0.0;
+                if (col->type == COL_SPHERE) {
+                    r = col->radius;
+                } else {
+                    for (uint32_t i = 0; i < col->h.nv; ++i) {
+                        real l = len(col->h.v[i]);
+                        if (l > r) r = l;
+                    }
+                }
+                if (r > maxr) maxr = r;
             }
-            e->radius = maxd > -DBL_MAX ? maxd : -DBL_MAX;
+            e->radius = maxr;
             e->fixed = d->body_fixed[g] != 0;
             if (e->fixed) {
                 e->inv_mass = 0.0;
@@ -1600,21 +1696,39 @@ orc_world* orc_world_create(const orc_world_desc* d) {
             } else {
                 real mass = d->body_mass[g];
                 e->inv_mass = 1.0 / mass;
-                /* colliders_get_default_inertia_tensor (collider.cpp:448-494) */
-                real mpv = mass / e->h.nv;
+                /* colliders_get_default_inertia_tensor (collider.cpp:448-494): a
+                 * lone sphere is a solid ball; otherwise the mass is spread
+                 * evenly over the vertices of all hull colliders (a sphere
+                 * among several colliders is an assert in the reference) */
                 m3 I;
                 memset(&I, 0, sizeof(I));
-                for (uint32_t i = 0; i < e->h.nv; ++i) {
-                    v3 v = e->h.v[i];
-                    I.m[0][0] += mpv * (v.y * v.y + v.z * v.z);
-                    I.m[0][1] += mpv * v.x * v.y;
-                    I.m[0][2] += mpv * v.x * v.z;
-                    I.m[1][0] += mpv * v.x * v.y;
-                    I.m[1][1] += mpv * (v.x * v.x + v.z * v.z);
-                    I.m[1][2] += mpv * v.y * v.z;
-                    I.m[2][0] += mpv * v.x * v.z;
-                    I.m[2][1] += mpv * v.y * v.z;
-                    I.m[2][2] += mpv * (v.x * v.x + v.y * v.y);
+                if (e->ncol == 1 && e->col[0].type == COL_SPHERE) {
+                    real r = e->col[0].radius;
+                    real Id = (2.0 / 5.0) * mass * r * r;
+                    I.m[0][0] = Id; I.m[1][1] = Id; I.m[2][2] = Id;
+                } else {
+                    uint32_t total = 0;
+                    for (uint32_t ci = 0; ci < e->ncol; ++ci) {
+                        if (e->col[ci].type == COL_SPHERE) sc->err |= ORC_ERR_HULL;
+                        else total += e->col[ci].h.nv;
+                    }
+                    real mpv = mass / (real)(double)total;
+                    for (uint32_t ci = 0; ci < e->ncol; ++ci) {
+                        if (e->col[ci].type == COL_SPHERE) continue;
+                        const hull* h = &e->col[ci].h;
+                        for (uint32_t i = 0; i < h->nv; ++i) {
+                            v3 v = h->v[i];
+                            I.m[0][0] += mpv * (v.y * v.y + v.z * v.z);
+                            I.m[0][1] += mpv * v.x * v.y;
+                            I.m[0][2] += mpv * v.x * v.z;
+                            I.m[1][0] += mpv * v.x * v.y;
+                            I.m[1][1] += mpv * (v.x * v.x + v.z * v.z);
+                            I.m[1][2] += mpv * v.y * v.z;
+                            I.m[2][0] += mpv * v.x * v.z;
+                            I.m[2][1] += mpv * v.y * v.z;
+                            I.m[2][2] += mpv * (v.x * v.x + v.y * v.y);
+                        }
+                    }
                 }
                 e->inertia = I;
                 if (!m3_inverse(&e->inertia, &e->inv_inertia)) sc->err |= ORC_ERR_HULL;
@@ -1673,7 +1787,12 @@ void orc_world_destroy(orc_world* w) {
     free(w->log_pairs.d);
     free(w->log_contacts.d);
     for (uint32_t si = 0; si < w->ns; ++si) {
-        for (uint32_t k = 0; k < w->s[si].nb; ++k) hull_free(&w->s[si].b[k].h);
+        for (uint32_t k = 0; k < w->s[si].nb; ++k) {
+            body* e = &w->s[si].b[k];
+            for (uint32_t ci = 0; ci < e->ncol; ++ci)
+                if (e->col[ci].type == COL_HULL) hull_free(&e->col[ci].h);
+            free(e->col);
+        }
         free(w->s[si].b);
         free(w->s[si].ext);
     }
@@ -1779,11 +1898,11 @@ uint32_t orc_world_contacts(orc_world* w, uint32_t si, uint32_t i, uint32_t j, d
     scene* sc = &w->s[si];
     if (i >= sc->nb || j >= sc->nb) return 0;
     body *e1 = &sc->b[i], *e2 = &sc->b[j];
-    hull_transform(&e1->h, e1->x, e1->q);
-    hull_transform(&e2->h, e2->x, e2->q);
+    colliders_update(e1);
+    colliders_update(e2);
     contactarr cts = {0};
     uint64_t fails = 0;
-    pair_contacts(&e1->h, &e2->h, &cts, &sc->err, &fails);
+    pair_contacts(e1, e2, &cts, &sc->err, &fails);
     for (uint32_t k = 0; k < cts.n && k < cap; ++k) {
         const contact* c = &cts.d[k];
         double* o = out + 9 * k;
@@ -1802,7 +1921,9 @@ void orc_world_hull(const orc_world* w, uint32_t body_idx, uint32_t counts[6], d
     uint32_t g = 0;
     for (uint32_t si = 0; si < w->ns; ++si) {
         if (body_idx < g + w->s[si].nb) {
-            const hull* h = &w->s[si].b[body_idx - g].h;
+            const body* e = &w->s[si].b[body_idx - g];
+            if (e->col[0].type != COL_HULL) break;
+            const hull* h = &e->col[0].h;  /* the body's first collider */
             counts[0] = h->nv; counts[1] = h->nf;
             counts[2] = h->fe_off[h->nf]; counts[3] = h->vf_off[h->nv];
             counts[4] = h->vn_off[h->nv]; counts[5] = h->ff_off[h->nf];

@@ -71,6 +71,14 @@ typedef struct {
     const uint32_t* scene_constraint_offset; /* [num_scenes+1] or NULL */
     const orc_constraint_desc* constraints;  /* external constraints, scene-grouped */
     uint32_t large_scene_bodies;             /* the device path's large-scene threshold (unused here) */
+    /* Spheres and compound bodies (NULL: every shape is a hull, every body
+     * has the one collider body_shape[g]). A shape with a radius > 0 is a
+     * sphere collider of that radius (collider_sphere_create, collider.cpp:12-18;
+     * body_scale does not apply). Body g's colliders are
+     * collider_shape[body_collider_offset[g] .. body_collider_offset[g+1]). */
+    const float* shape_sphere_radius;        /* [num_shapes] or NULL */
+    const uint32_t* body_collider_offset;    /* [nb+1] or NULL */
+    const uint32_t* collider_shape;          /* [colliders] */
 } orc_world_desc;
 
 /* Per-body state, read and written as flat arrays (any pointer may be NULL). */

@@ -199,7 +199,8 @@ class WorldDescC(ctypes.Structure):
                 ("body_scale", _P), ("body_position", _P), ("body_rotation", _P),
                 ("body_linear_velocity", _P), ("body_angular_velocity", _P), ("body_mass", _P),
                 ("body_fixed", _P), ("body_material", _P), ("scene_constraint_offset", _P),
-                ("constraints", _P), ("large_scene_bodies", ctypes.c_uint32)]
+                ("constraints", _P), ("large_scene_bodies", ctypes.c_uint32), ("shape_sphere_radius", _P),
+                ("body_collider_offset", _P), ("collider_shape", _P)]
 
 
 class StateC(ctypes.Structure):
@@ -242,18 +243,41 @@ class World:
     constraints: list = field(default_factory=list)      # ConstraintDesc
     name: str = ""
     large_scene_bodies: int = 0                          # device large-scene threshold (0: library default)
+    sphere_radius: list = field(default_factory=list)    # per shape: > 0 for a sphere collider
+    body_colliders: list = field(default_factory=list)   # per body: tuple of shapes (compound) or None
 
     # -- building -----------------------------------------------------------
+    def sphere(self, radius: float) -> int:
+        """A sphere collider shape (collider_sphere_create); its radius is a
+        float as in the reference (collider.h:29)."""
+        r = float(np.float32(radius))
+        for i, (s, sr) in enumerate(zip(self.shapes, self.sphere_radius)):
+            if sr == r and len(s) == 0:
+                return i
+        self.shapes.append(np.zeros((0, 3), dtype=np.float32))
+        self.sphere_radius.append(r)
+        return len(self.shapes) - 1
+
     def shape(self, corners: np.ndarray) -> int:
+        while len(self.sphere_radius) < len(self.shapes):
+            self.sphere_radius.append(0.0)
         for i, s in enumerate(self.shapes):
             if s is corners or (s.shape == corners.shape and np.array_equal(s, corners)):
                 return i
         self.shapes.append(np.ascontiguousarray(corners, dtype=np.float32))
+        self.sphere_radius.append(0.0)
         return len(self.shapes) - 1
 
     def add_body(self, shape: int, scale, position, rotation, mass=1.0, fixed=False,
                  material=(0.5, 0.3, 0.0), linear_velocity=(0.0, 0.0, 0.0),
                  angular_velocity=(0.0, 0.0, 0.0)) -> int:
+        """`shape` is one shape index, or a sequence of them for a compound
+        body (one collider per entry, in order)."""
+        if isinstance(shape, (list, tuple)):
+            self.body_colliders.append(tuple(int(k) for k in shape))
+            shape = shape[0]
+        else:
+            self.body_colliders.append(None)
         self.body_shape.append(shape)
         self.body_scale.append(tuple(scale))
         self.body_position.append(tuple(position))
@@ -305,6 +329,15 @@ class World:
         }
         if self.constraints:
             a["constraints"] = (ConstraintDesc * len(self.constraints))(*self.constraints)
+        if any(r > 0.0 for r in self.sphere_radius):
+            a["shape_sphere_radius"] = np.asarray(self.sphere_radius, dtype=np.float32)
+        if any(c is not None for c in self.body_colliders):
+            offs, shapes = [0], []
+            for b, c in enumerate(self.body_colliders):
+                shapes.extend(c if c is not None else (self.body_shape[b],))
+                offs.append(len(shapes))
+            a["body_collider_offset"] = np.asarray(offs, dtype=np.uint32)
+            a["collider_shape"] = np.asarray(shapes, dtype=np.uint32)
         return a
 
     def desc(self):
@@ -318,7 +351,10 @@ class World:
             a["body_linear_velocity"].ctypes.data, a["body_angular_velocity"].ctypes.data,
             a["body_mass"].ctypes.data, a["body_fixed"].ctypes.data, a["body_material"].ctypes.data,
             a["scene_constraint_offset"].ctypes.data if cons is not None else None,
-            ctypes.addressof(cons) if cons is not None else None, self.large_scene_bodies)
+            ctypes.addressof(cons) if cons is not None else None, self.large_scene_bodies,
+            a["shape_sphere_radius"].ctypes.data if "shape_sphere_radius" in a else None,
+            a["body_collider_offset"].ctypes.data if "body_collider_offset" in a else None,
+            a["collider_shape"].ctypes.data if "collider_shape" in a else None)
         return d, a
 
 
@@ -391,6 +427,54 @@ def batched_uniform(num_scenes: int, first_seed: int = 0, **kw) -> World:
     w = World(name=f"batched_uniform_{num_scenes}")
     for s in range(num_scenes):
         drop_scene(w, first_seed + s, **kw)
+    return w
+
+
+def box_corners(half, offset=(0.0, 0.0, 0.0)) -> np.ndarray:
+    """cube.obj's corner stream scaled by `half` and moved by `offset`: a box
+    collider placed off the body origin (a part of a compound body)."""
+    c = cube_corners().astype(np.float64) * np.asarray(half, dtype=np.float64) + np.asarray(offset)
+    return c.astype(np.float32)
+
+
+def mixed_scene(w: World, seed: int, n_spheres: int = 5, n_cubes: int = 3, n_dumbbells: int = 2) -> World:
+    """Sphere colliders and compound bodies: a static floor, spheres of radius
+    U(0.5, 1.0), unit cubes, and dumbbells (two 0.5-half-extent boxes at
+    x = +-0.75, one body with two hull colliders), dropped from a 4 x 4 lattice
+    with lowest points U(1, 5) above the floor and random orientation and
+    velocity N(0, 0.5). Masses are density-1 volumes (4/3 pi r^3 for spheres).
+    Friction 0.5/0.3, restitution U(0, 0.3)."""
+    rng = Rng(0x5EED0000 + seed)
+    _floor(w)
+    cube = w.shape(cube_corners())
+    left = w.shape(box_corners((0.5, 0.5, 0.5), (-0.75, 0.0, 0.0)))
+    right = w.shape(box_corners((0.5, 0.5, 0.5), (0.75, 0.0, 0.0)))
+    n = n_spheres + n_cubes + n_dumbbells
+    for k in range(n):
+        gx, gz = k % 4, (k // 4) % 4
+        layer = k // 16
+        x = (gx - 1.5) * GRID_SPACING
+        z = (gz - 1.5) * GRID_SPACING
+        q = rng.quaternion()
+        v = (rng.normal(0.5), rng.normal(0.5), rng.normal(0.5))
+        e = rng.uniform(0.0, 0.3)
+        if k < n_spheres:
+            r = rng.uniform(0.5, 1.0)
+            shape, bound, mass = w.sphere(r), r, 4.0 / 3.0 * math.pi * r ** 3
+        elif k < n_spheres + n_cubes:
+            shape, bound, mass = cube, math.sqrt(3.0), 8.0
+        else:
+            shape, bound, mass = (left, right), math.sqrt(1.25 ** 2 + 0.5), 2.0
+        y = -1.0 + bound + rng.uniform(1.0, 5.0) + 8.0 * layer
+        w.add_body(shape, (1.0, 1.0, 1.0), (x, y, z), q, mass=mass, material=(0.5, 0.3, e), linear_velocity=v)
+    w.end_scene()
+    return w
+
+
+def batched_mixed(num_scenes: int, first_seed: int = 0, **kw) -> World:
+    w = World(name=f"batched_mixed_{num_scenes}")
+    for s in range(num_scenes):
+        mixed_scene(w, first_seed + s, **kw)
     return w
 
 

@@ -46,6 +46,7 @@ CASES = {
     "chains_nocoll": (lambda: scenes.chains(2, length=6), [1, 60, 300], {"collisions": False}),
     "chains": (lambda: scenes.chains(2, length=6, first_seed=10), [1, 60, 300], {}),
     "iters4": (lambda: scenes.batched_uniform(2, first_seed=200), [1, 60], {"iters": 4, "substeps": 10}),
+    "mixed": (lambda: scenes.batched_mixed(4, first_seed=1), [1, 30, 120, 300], {}),
 }
 
 

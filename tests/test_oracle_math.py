@@ -178,3 +178,51 @@ def test_resting_body_sleeps_after_one_second():
     # quiet frames accumulate dt per frame; sleep needs >= 1.0 s of them
     assert 60 <= f + 1 <= 90
     assert st["deactivation_time"][1, 0] >= 1.0
+
+
+def test_sphere_and_compound_inertia_closed_forms():
+    """A lone sphere is a solid ball, I = (2/5) m r^2 with r the float radius
+    (collider.cpp:452-462); a compound spreads its mass over the vertices of
+    all its hulls (collider.cpp:465-491): two half-extent-0.5 boxes at
+    x = +-0.75 give 16 corner masses m/16 at (+-0.75 +- 0.5, +-0.5, +-0.5)."""
+    w = scenes.World(name="inertia")
+    r = 0.7
+    sph = w.sphere(r)
+    left = w.shape(scenes.box_corners((0.5, 0.5, 0.5), (-0.75, 0.0, 0.0)))
+    right = w.shape(scenes.box_corners((0.5, 0.5, 0.5), (0.75, 0.0, 0.0)))
+    w.add_body(sph, (1, 1, 1), (0, 0, 0), (0, 0, 0, 1), mass=3.0)
+    w.add_body((left, right), (1, 1, 1), (0, 5, 0), (0, 0, 0, 1), mass=2.0)
+    w.end_scene()
+    _, inertia, _, radius = SimWorld("orc", w).body_info()
+    rf = float(np.float32(r))
+    assert np.allclose(inertia[0].reshape(3, 3), np.eye(3) * 0.4 * 3.0 * rf * rf, rtol=1e-15, atol=0)
+    assert radius[0] == rf
+    xs = np.array([x + d for x in (-0.75, 0.75) for d in (-0.5, 0.5)])
+    pts = np.array([(x, y, z) for x in xs for y in (-0.5, 0.5) for z in (-0.5, 0.5)])
+    m = 2.0 / len(pts)
+    I = np.zeros((3, 3))
+    for x, y, z in pts:
+        I += m * np.array([[y * y + z * z, x * y, x * z], [x * y, x * x + z * z, y * z], [x * z, y * z, x * x + y * y]])
+    assert np.allclose(inertia[1].reshape(3, 3), I, rtol=1e-12, atol=1e-15)
+    assert np.isclose(radius[1], np.sqrt(1.25 ** 2 + 0.5), rtol=1e-15)
+
+
+def test_sphere_rests_at_its_radius():
+    """A ball dropped on the floor (top face y = -1) comes to rest with its
+    centre at y = -1 + r, and its single contact sits at the ball's bottom."""
+    w = scenes.World(name="ball")
+    scenes._floor(w)
+    r = 0.8
+    w.add_body(w.sphere(r), (1, 1, 1), (0.2, 1.0, -0.1), (0, 0, 0, 1), mass=1.0, material=(0.5, 0.3, 0.0))
+    w.end_scene()
+    sim = SimWorld("orc", w)
+    sim.step(180)
+    y = sim.state()["position"][1, 1]
+    assert abs(y - (-1.0 + float(np.float32(r)))) < 2e-3
+    st = sim.state()
+    st["position"][1, 1] -= 0.005
+    sim.set_state(st)
+    c = sim.contacts(0, 0, 1)
+    assert len(c) == 1
+    p_ball = c[0, 3:6]
+    assert np.allclose(p_ball, st["position"][1] - np.array([0.0, float(np.float32(r)), 0.0]), atol=1e-9)
