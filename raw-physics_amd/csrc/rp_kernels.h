@@ -35,7 +35,15 @@ struct DevWorld {
     const int32_t* topo_pool;
     // transformed geometry
     V3* tv;
-    const int2* xf;                 // [n_xf] (body, vertex)
+    const int4* xf;                 // [n_xf] (body, hull instance, vertex, tv slot)
+    // colliders (bodies with one hull collider take the fast narrow path)
+    int has_general;                // some body has a sphere or several colliders
+    const int32_t* body_col_off;    // [nb+1]
+    const int32_t* col_type;        // [colliders] COL_HULL / COL_SPHERE
+    const int32_t* col_hull;        // [colliders] hull instance
+    const int32_t* col_tv_off;      // [colliders] first transformed vertex
+    const float* col_radius;        // [colliders] sphere radius (r32)
+    const int32_t* body_general;    // [nb]
     int32_t* need;                  // [nb] body takes part in a live pair this frame
     // pairs
     int2* pairs;                    // [pair capacity] scene-local (i, j)

@@ -205,19 +205,18 @@ __device__ __forceinline__ V3 transform_vertex(const M4& M, V3 v) {
 __global__ void k_transform(DevWorld w, int all) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= w.n_xf) return;
-    int2 el = w.xf[t];
+    int4 el = w.xf[t];  // (body, hull instance, vertex, tv slot)
     int b = el.x;
     if (!all && !w.need[b]) return;
-    int hid = w.body_hull[b];
-    w.tv[w.body_tv_off[b] + el.y] = transform_vertex(w.model[b], w.hull_v[w.hull_v_off[hid] + el.y]);
+    w.tv[el.w] = transform_vertex(w.model[b], w.hull_v[w.hull_v_off[el.y] + el.z]);
 }
 
-__device__ __forceinline__ HullView hull_view(const DevWorld& w, int b) {
+// View of hull instance `hid` of body b whose transformed vertices start at tv_off.
+__device__ __forceinline__ HullView hull_view_of(const DevWorld& w, int b, int hid, int tv_off) {
     HullView hv;
-    int hid = w.body_hull[b];
     int tp = w.hull_topo[hid];
     const int32_t* tb = w.topo_base + 8 * tp;
-    hv.tv = w.tv + w.body_tv_off[b];
+    hv.tv = w.tv + tv_off;
     hv.fn = w.hull_fn + w.hull_fn_off[hid];
     const M4& M = w.model[b];
 #pragma unroll
@@ -236,6 +235,20 @@ __device__ __forceinline__ HullView hull_view(const DevWorld& w, int b) {
     hv.ff = w.topo_pool + tb[7];
     return hv;
 }
+// The hull of a body with one hull collider (the fast narrow path).
+__device__ __forceinline__ HullView hull_view(const DevWorld& w, int b) {
+    return hull_view_of(w, b, w.body_hull[b], w.body_tv_off[b]);
+}
+// Collider c (global index) of body b.
+__device__ __forceinline__ ColView col_view(const DevWorld& w, int b, int c) {
+    ColView v;
+    v.type = w.col_type[c];
+    v.radius = w.col_radius[c];
+    v.center = w.body[b].x;  // colliders_update: the sphere follows the entity (collider.cpp:431-433)
+    if (v.type == COL_HULL) v.h = hull_view_of(w, b, w.col_hull[c], w.col_tv_off[c]);
+    else memset(&v.h, 0, sizeof(v.h));
+    return v;
+}
 
 // Narrow phase over the live pairs of all scenes (pbd.cpp:583-612), in two
 // kernels: GJK + EPA for every live pair, then the contact manifold for the
@@ -251,6 +264,20 @@ __device__ __forceinline__ void group_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Scene and global body indices of live pair g.
+__device__ __forceinline__ void pair_bodies(const DevWorld& w, int g, int& s, int& b1, int& b2) {
+    int2 sp = w.live_global[g];
+    s = sp.x;
+    int2 p = w.pairs[sp.y];
+    int b0 = w.scene_body_off[s];
+    b1 = b0 + p.x;
+    b2 = b0 + p.y;
+}
+// Pairs with a sphere or a multi-collider body take the general path.
+__device__ __forceinline__ bool general_pair(const DevWorld& w, int b1, int b2) {
+    return w.has_general && (w.body_general[b1] | w.body_general[b2]);
 }
 
 // Hull views of live pair g; for G > 1 the group's lanes transform both
@@ -286,6 +313,8 @@ __global__ void __launch_bounds__(64) k_gjk_epa(DevWorld w) {
     int total = w.live_prefix[w.ns];
     for (int g = blockIdx.x * NG + grp; g < total; g += gridDim.x * NG) {
         int s, b1, b2;
+        pair_bodies(w, g, s, b1, b2);
+        if (general_pair(w, b1, b2)) continue;  // k_narrow_general (uniform per group)
         HullView h1, h2;
         pair_views<G>(w, g, st, gl, s, b1, b2, h1, h2);
         uint32_t err = 0;
@@ -335,6 +364,35 @@ __global__ void __launch_bounds__(64) k_manifold(DevWorld w) {
             if (err) atomicOr(&w.scene_err[s], err);
         }
         if constexpr (G > 1) group_lds_sync();
+    }
+}
+
+// Live pairs with a sphere collider or a multi-collider body, one lane per
+// pair: colliders_get_contacts (collider.cpp:560-572) over every collider
+// pair in order, each by collider_get_contacts (collider.cpp:523-558).
+__global__ void __launch_bounds__(64) k_narrow_general(DevWorld w) {
+    int total = w.live_prefix[w.ns];
+    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += gridDim.x * blockDim.x) {
+        int s, b1, b2;
+        pair_bodies(w, g, s, b1, b2);
+        if (!general_pair(w, b1, b2)) continue;
+        uint32_t err = 0;
+        int flags = 0;
+        ContactPt cts[MAXC];
+        int nc = 0;
+        for (int c1 = w.body_col_off[b1]; c1 < w.body_col_off[b1 + 1]; ++c1)
+            for (int c2 = w.body_col_off[b2]; c2 < w.body_col_off[b2 + 1]; ++c2) {
+                ColView v1 = col_view(w, b1, c1), v2 = col_view(w, b2, c2);
+                nc += collider_contacts(v1, v2, cts + nc, MAXC - nc, err, flags);
+            }
+        const Body& e1 = w.body[b1];
+        const Body& e2 = w.body[b2];
+        Contact* out = w.contacts + (size_t)g * MAXC;
+        for (int c = 0; c < nc; ++c) contact_to_constraint(e1, e2, cts[c], out[c]);
+        w.ncontact[g] = nc;
+        if (err) atomicOr(&w.scene_err[s], err);
+        if (flags & PAIR_EPA_FAIL) atomicAdd(&w.stats[STAT_EPA_FAIL], 1ull);
+        if (flags & PAIR_GJK_CYCLE) atomicAdd(&w.stats[STAT_GJK_CYCLE], 1ull);
     }
 }
 
@@ -576,7 +634,7 @@ hipError_t launch_frame(const DevWorld& w, const FrameParams& fp, hipStream_t st
         hipLaunchKernelGGL(k_integrate, dim3(cdiv(w.nb, 128)), dim3(128), 0, st, w, fp.h);
         t.end(st);
         if (fp.collisions) {
-            if (w.narrow_group == 1 || w.manifold_group == 1) {
+            if (w.narrow_group == 1 || w.manifold_group == 1 || w.has_general) {
                 t.begin(KC_TRANSFORM, st);
                 hipLaunchKernelGGL(k_transform, dim3(cdiv(w.n_xf, 256)), dim3(256), 0, st, w, 0);
                 t.end(st);
@@ -590,6 +648,7 @@ hipError_t launch_frame(const DevWorld& w, const FrameParams& fp, hipStream_t st
             case 16: hipLaunchKernelGGL(k_gjk_epa<16>, nb, t64, lds, st, w); break;
             default: hipLaunchKernelGGL(k_gjk_epa<1>, nb, t64, 0, st, w); break;
             }
+            if (w.has_general) hipLaunchKernelGGL(k_narrow_general, dim3(1024), t64, 0, st, w);
             t.end(st);
             t.begin(KC_MANIFOLD, st);
             lds = w.manifold_group > 1 ? (size_t)(64 / w.manifold_group) * w.stage_cap * sizeof(V3) : 0;

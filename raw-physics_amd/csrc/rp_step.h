@@ -55,6 +55,17 @@ struct HullView {
     const int32_t *fe_off, *fe, *vf_off, *vf, *vn_off, *vn, *ff_off, *ff;
 };
 
+// A collider of a body with more than one collider or a sphere (the general
+// narrow-phase path): a hull view, or a sphere whose center is the body
+// position (collider.cpp:431-433) with an r32 radius (collider.h:29).
+enum { COL_HULL = 0, COL_SPHERE = 1 };
+struct ColView {
+    int type;
+    HullView h;
+    V3 center;
+    float radius;
+};
+
 // gm_mat4_multiply_vec3(M, n, false) then normalize (collider.cpp:426-428)
 RP_HD V3 face_normal_world(const HullView& h, int f) {
     V3 n = h.fn[f];
@@ -181,6 +192,20 @@ RP_HD V3 support_md(const HullView& a, const HullView& b, V3 d, uint32_t& err) {
     return sub(a.tv[i1], b.tv[i2]);
 }
 
+// support_point of one collider (support.cpp:19-31); the general path runs
+// one lane per pair.
+RP_HD V3 support_point(const ColView& c, V3 d, uint32_t& err) {
+    if (c.type == COL_SPHERE) return add(c.center, scl((double)c.radius, nrm(d)));
+    return c.h.tv[support_index<1>(c.h, d, err)];
+}
+// support_point_of_minkowski_difference (support.cpp:34-39)
+template <int G = 1>
+RP_HD V3 support_md(const ColView& a, const ColView& b, V3 d, uint32_t& err) {
+    V3 s1 = support_point(a, d, err);
+    V3 s2 = support_point(b, scl(-1.0, d), err);
+    return sub(s1, s2);
+}
+
 // ---------------------------------------------------------------------------
 // GJK (gjk.cpp:7-379)
 // ---------------------------------------------------------------------------
@@ -296,8 +321,8 @@ RP_HD bool same_state(const Simplex& s, V3 dir, const Simplex& t, V3 tdir) {
 // every later state; a match means the iterations since the save returned
 // nothing and would repeat until the limit (their error bits are already
 // set), so false is returned at once. `cycled` is set when this happens.
-template <int G = 1>
-RP_HD bool gjk(const HullView& h1, const HullView& h2, Simplex& s, uint32_t& err, bool& cycled) {
+template <int G = 1, class View = HullView>
+RP_HD bool gjk(const View& h1, const View& h2, Simplex& s, uint32_t& err, bool& cycled) {
     s.a = support_md<G>(h1, h2, v3(0.0, 0.0, 1.0), err);
     s.b = s.c = s.d = v3(0.0, 0.0, 0.0);
     s.num = 1;
@@ -369,8 +394,8 @@ RP_HD bool epa_face(I3 f, const V3* poly, int np, V3& n_out, double& d_out) {
 }
 
 // epa (epa.cpp:118-237): 0 = no convergence, 1 = converged, -1 = error
-template <int G = 1>
-RP_HD int epa(const HullView& h1, const HullView& h2, const Simplex& s, V3& n_out, double& pen_out, uint32_t& err) {
+template <int G = 1, class View = HullView>
+RP_HD int epa(const View& h1, const View& h2, const Simplex& s, V3& n_out, double& pen_out, uint32_t& err) {
     V3 poly[EPA_MAXV];
     I3 faces[EPA_MAXF];
     V3 normals[EPA_MAXF];
@@ -648,6 +673,67 @@ RP_HD int pair_contacts(const HullView& h1, const HullView& h2, ContactPt* out, 
     int r = epa<G>(h1, h2, s, n, pen, err);
     if (r != 1) { flags |= PAIR_EPA_FAIL; return 0; }
     return manifold<G>(h1, h2, n, out, err);
+}
+
+// clipping_get_contact_manifold (clipping.cpp:343-371) for colliders: a
+// sphere gives one contact at its support point along the normal.
+RP_HD int collider_manifold(const ColView& c1, const ColView& c2, V3 normal, double penetration, ContactPt* out,
+                            uint32_t& err) {
+    if (c1.type == COL_SPHERE) {
+        V3 p = support_point(c1, normal, err);
+        out[0].p1 = p;
+        out[0].p2 = sub(p, scl(penetration, normal));
+        out[0].n = normal;
+        return 1;
+    }
+    if (c2.type == COL_SPHERE) {
+        V3 inormal = inv3(normal);
+        V3 p = support_point(c2, inormal, err);
+        out[0].p1 = add(p, scl(penetration, normal));
+        out[0].p2 = p;
+        out[0].n = normal;
+        return 1;
+    }
+    return manifold<1>(c1.h, c2.h, normal, out, err);
+}
+
+// collider_get_contacts (collider.cpp:523-558) for any two colliders. Two
+// spheres are resolved with the reference's float arithmetic; the float
+// square root is taken in double and rounded once, which is the correctly
+// rounded result (a 53-bit square root of a 24-bit value). `cap` is the room
+// left in `out`.
+RP_HD int collider_contacts(const ColView& c1, const ColView& c2, ContactPt* out, int cap, uint32_t& err, int& flags) {
+    ContactPt tmp[MAXC];
+    int n = 0;
+    if (c1.type == COL_SPHERE && c2.type == COL_SPHERE) {
+        V3 dv = sub(c1.center, c2.center);
+        float dsq = (float)dot(dv, dv);
+        float mind = c1.radius + c2.radius;
+        if (dsq < (mind * mind)) {
+            V3 nrm_ = nrm(sub(c2.center, c1.center));
+            float root = (float)sqrt((double)dsq);
+            double pen = (double)(mind - root);
+            n = collider_manifold(c1, c2, nrm_, pen, tmp, err);
+        }
+    } else {
+        Simplex s;
+        bool cycled = false;
+        if (gjk<1>(c1, c2, s, err, cycled)) {
+            V3 nn;
+            double pen;
+            int r = epa<1>(c1, c2, s, nn, pen, err);
+            if (r != 1) flags |= PAIR_EPA_FAIL;
+            else n = collider_manifold(c1, c2, nn, pen, tmp, err);
+        } else if (cycled) {
+            flags |= PAIR_GJK_CYCLE;
+        }
+    }
+    if (n > cap) {
+        err |= ERR_CONTACT_CAPACITY;
+        n = cap;
+    }
+    for (int k = 0; k < n; ++k) out[k] = tmp[k];
+    return n;
 }
 
 // ---------------------------------------------------------------------------

@@ -192,6 +192,13 @@ static HostHull build_hull(const std::vector<V3>& corners) {
     return H;
 }
 
+// One collider of an entity (collider.h:27-44): a hull instance or a sphere.
+struct HostCollider {
+    int32_t type;   // COL_HULL or COL_SPHERE
+    int32_t hull;   // hull instance (COL_HULL)
+    float radius;   // r32 radius (COL_SPHERE, collider.h:29)
+};
+
 // Point-mass inertia at the hull vertices (colliders_get_default_inertia_tensor,
 // collider.cpp:448-494) and its inverse (gm_mat3_inverse, gm.h:347-368).
 static M3 hull_inertia(const HostHull& h, double mass) {
@@ -237,6 +244,52 @@ static double hull_radius(const HostHull& h) {  // collider.cpp:178-189, 510-521
     return maxd;
 }
 
+// colliders_get_bounding_sphere_radius (collider.cpp:510-521)
+static double colliders_radius(const std::vector<HostHull>& hulls, const HostCollider* c, int n) {
+    double r = -DBL_MAX;
+    for (int i = 0; i < n; ++i) {
+        double ri = (c[i].type == COL_SPHERE) ? (double)c[i].radius : hull_radius(hulls[c[i].hull]);
+        if (ri > r) r = ri;
+    }
+    return r;
+}
+
+// colliders_get_default_inertia_tensor (collider.cpp:448-494): a lone sphere
+// is a solid ball; otherwise the mass is spread evenly over the vertices of
+// every hull collider (a sphere among several colliders is an assert there).
+static M3 colliders_inertia(const std::vector<HostHull>& hulls, const HostCollider* c, int n, double mass,
+                            uint32_t& err) {
+    M3 I;
+    memset(&I, 0, sizeof(I));
+    if (n == 1 && c[0].type == COL_SPHERE) {
+        double Id = (2.0 / 5.0) * mass * (double)c[0].radius * (double)c[0].radius;
+        I.m[0][0] = I.m[1][1] = I.m[2][2] = Id;
+        return I;
+    }
+    if (n == 1) return hull_inertia(hulls[c[0].hull], mass);
+    uint32_t total = 0;
+    for (int i = 0; i < n; ++i) {
+        if (c[i].type == COL_SPHERE) err |= ERR_HULL;
+        else total += (uint32_t)hulls[c[i].hull].v.size();
+    }
+    double mpv = mass / total;
+    for (int i = 0; i < n; ++i) {
+        if (c[i].type == COL_SPHERE) continue;
+        for (const V3& v : hulls[c[i].hull].v) {
+            I.m[0][0] += mpv * (v.y * v.y + v.z * v.z);
+            I.m[0][1] += mpv * v.x * v.y;
+            I.m[0][2] += mpv * v.x * v.z;
+            I.m[1][0] += mpv * v.x * v.y;
+            I.m[1][1] += mpv * (v.x * v.x + v.z * v.z);
+            I.m[1][2] += mpv * v.y * v.z;
+            I.m[2][0] += mpv * v.x * v.z;
+            I.m[2][1] += mpv * v.y * v.z;
+            I.m[2][2] += mpv * (v.x * v.x + v.y * v.y);
+        }
+    }
+    return I;
+}
+
 // ---------------------------------------------------------------------------
 // Host-side world and its device image
 // ---------------------------------------------------------------------------
@@ -251,7 +304,9 @@ struct HostWorld {
     uint32_t large_scene_bodies = kLargeSceneBodies;
     std::vector<int32_t> scene_body_off{0}, scene_ext_off{0};
     std::vector<Body> body;
-    std::vector<int32_t> body_hull;
+    std::vector<int32_t> body_hull;     // first collider's hull instance (-1: a sphere)
+    std::vector<int32_t> body_col_off{0};
+    std::vector<HostCollider> cols;     // every body's colliders, in order
     std::vector<HostHull> hulls;
     std::vector<ExtC> ext;
     std::vector<uint32_t> scene_err;  // errors found while building
@@ -330,17 +385,33 @@ static bool build_image(const HostWorld& hw, DeviceImage& im) {
         hull_v.insert(hull_v.end(), h.v.begin(), h.v.end());
         hull_fn.insert(hull_fn.end(), h.fn.begin(), h.fn.end());
     }
-    // per-body transformed geometry and the transform work list
-    std::vector<int2> xf;
+    // per-collider transformed geometry and the transform work list
+    // (body, hull, vertex, tv slot); a body's first hull collider's vertices
+    // start at body_tv_off
+    std::vector<int4> xf;
+    std::vector<int32_t> col_type, col_hull, col_tv_off, body_general(d.nb, 0);
+    std::vector<float> col_radius;
     int64_t ntv = 0;
     im.body_tv_off.resize(d.nb);
     for (int b = 0; b < d.nb; ++b) {
-        const HostHull& h = hw.hulls[hw.body_hull[b]];
+        int c0 = hw.body_col_off[b], c1 = hw.body_col_off[b + 1];
         im.body_tv_off[b] = (int32_t)ntv;
-        for (size_t i = 0; i < h.v.size(); ++i) xf.push_back(make_int2(b, (int)i));
-        ntv += (int64_t)h.v.size();
+        if (c1 - c0 != 1 || hw.cols[c0].type != COL_HULL) body_general[b] = 1;
+        for (int c = c0; c < c1; ++c) {
+            const HostCollider& hc = hw.cols[c];
+            col_type.push_back(hc.type);
+            col_hull.push_back(hc.hull);
+            col_radius.push_back(hc.radius);
+            col_tv_off.push_back((int32_t)ntv);
+            if (hc.type != COL_HULL) continue;
+            const HostHull& h = hw.hulls[hc.hull];
+            for (size_t i = 0; i < h.v.size(); ++i) xf.push_back(make_int4(b, hc.hull, (int)i, (int)(ntv + i)));
+            ntv += (int64_t)h.v.size();
+        }
     }
     d.n_xf = (int)xf.size();
+    d.has_general = 0;
+    for (int32_t g : body_general) d.has_general |= g;
     // pair and constraint capacities per scene; large scenes get a bounded
     // pair capacity (their broad phase reports ERR_PAIR_CAPACITY beyond it)
     std::vector<int32_t> pair_base(d.ns + 1, 0), con_base(d.ns + 1, 0), scene_big(d.ns, 0);
@@ -431,6 +502,12 @@ static bool build_image(const HostWorld& hw, DeviceImage& im) {
     ok &= im.alloc(d.tv, (size_t)ntv);
     ok &= im.alloc(d.need, d.nb);
     ok &= im.upload(d.xf, xf);
+    ok &= im.upload(d.body_col_off, hw.body_col_off);
+    ok &= im.upload(d.col_type, col_type);
+    ok &= im.upload(d.col_hull, col_hull);
+    ok &= im.upload(d.col_tv_off, col_tv_off);
+    ok &= im.upload(d.col_radius, col_radius);
+    ok &= im.upload(d.body_general, body_general);
     ok &= im.alloc(d.pairs, (size_t)pcap);
     ok &= im.alloc(d.pair_count, d.ns);
     ok &= im.alloc(d.live_local, (size_t)pcap);
@@ -489,16 +566,16 @@ static bool build_image(const HostWorld& hw, DeviceImage& im) {
 }
 
 // Entity fields at creation (entity_create_ex, entity.cpp:24-65).
-static Body make_body(const HostHull& h, V3 x, Q4 q, V3 v, V3 w, double mass, bool fixed, const double mat[3],
-                      uint32_t& err) {
+static Body make_body(const HostWorld& hw, int c0, int nc, V3 x, Q4 q, V3 v, V3 w, double mass, bool fixed,
+                      const double mat[3], uint32_t& err) {
     Body e;
     memset(&e, 0, sizeof(e));
     e.x = x; e.q = q; e.v = v; e.w = w;
-    e.radius = hull_radius(h);
+    e.radius = colliders_radius(hw.hulls, hw.cols.data() + c0, nc);
     e.fixed = fixed ? 1 : 0;
     if (!fixed) {
         e.inv_mass = 1.0 / mass;
-        e.I = hull_inertia(h, mass);
+        e.I = colliders_inertia(hw.hulls, hw.cols.data() + c0, nc, mass, err);
         if (!m3_inverse(e.I, e.Iinv)) err |= ERR_HULL;
     }
     e.active = 1;
@@ -518,36 +595,50 @@ static bool host_world_from_desc(const rp_world_desc* d, HostWorld& hw) {
     std::map<std::tuple<uint32_t, uint64_t, uint64_t, uint64_t>, int32_t> inst;
     for (uint32_t s = 0; s < ns; ++s) {
         for (uint32_t g = d->scene_body_offset[s]; g < d->scene_body_offset[s + 1]; ++g) {
-            uint32_t shp = d->body_shape[g];
-            if (shp >= d->num_shapes) { set_msg("body_shape out of range"); return false; }
             const double* sc = d->body_scale + 3 * g;
             uint64_t k0, k1, k2;
             memcpy(&k0, sc, 8); memcpy(&k1, sc + 1, 8); memcpy(&k2, sc + 2, 8);
-            auto key = std::make_tuple(shp, k0, k1, k2);
-            auto it = inst.find(key);
-            if (it == inst.end()) {
-                std::vector<V3> corners;
-                for (uint32_t c = d->shape_corner_offset[shp]; c < d->shape_corner_offset[shp + 1]; ++c) {
-                    // examples_util.cpp:7-16: float widened to double, then scaled
-                    V3 p = v3((double)d->shape_corners[3 * c], (double)d->shape_corners[3 * c + 1],
-                              (double)d->shape_corners[3 * c + 2]);
-                    p.x *= sc[0]; p.y *= sc[1]; p.z *= sc[2];
-                    corners.push_back(p);
+            uint32_t cb = d->body_collider_offset ? d->body_collider_offset[g] : 0;
+            uint32_t nc = d->body_collider_offset ? d->body_collider_offset[g + 1] - cb : 1;
+            int c0 = (int)hw.cols.size();
+            for (uint32_t ci = 0; ci < nc; ++ci) {
+                uint32_t shp = d->body_collider_offset ? d->collider_shape[cb + ci] : d->body_shape[g];
+                if (shp >= d->num_shapes) { set_msg("body shape out of range"); return false; }
+                HostCollider col = {COL_HULL, -1, 0.0f};
+                if (d->shape_sphere_radius && d->shape_sphere_radius[shp] > 0.0f) {
+                    col.type = COL_SPHERE;  // collider_sphere_create (collider.cpp:12-18)
+                    col.radius = d->shape_sphere_radius[shp];
+                    hw.cols.push_back(col);
+                    continue;
                 }
-                hw.hulls.push_back(build_hull(corners));
-                it = inst.emplace(key, (int32_t)hw.hulls.size() - 1).first;
+                auto key = std::make_tuple(shp, k0, k1, k2);
+                auto it = inst.find(key);
+                if (it == inst.end()) {
+                    std::vector<V3> corners;
+                    for (uint32_t c = d->shape_corner_offset[shp]; c < d->shape_corner_offset[shp + 1]; ++c) {
+                        // examples_util.cpp:7-16: float widened to double, then scaled
+                        V3 p = v3((double)d->shape_corners[3 * c], (double)d->shape_corners[3 * c + 1],
+                                  (double)d->shape_corners[3 * c + 2]);
+                        p.x *= sc[0]; p.y *= sc[1]; p.z *= sc[2];
+                        corners.push_back(p);
+                    }
+                    hw.hulls.push_back(build_hull(corners));
+                    it = inst.emplace(key, (int32_t)hw.hulls.size() - 1).first;
+                }
+                if (!hw.hulls[it->second].ok) hw.scene_err[s] |= ERR_HULL;
+                col.hull = it->second;
+                hw.cols.push_back(col);
             }
-            const HostHull& h = hw.hulls[it->second];
-            if (!h.ok) hw.scene_err[s] |= ERR_HULL;
+            hw.body_col_off.push_back((int32_t)hw.cols.size());
             const double* P = d->body_position + 3 * g;
             const double* R = d->body_rotation + 4 * g;
             const double* LV = d->body_linear_velocity + 3 * g;
             const double* AV = d->body_angular_velocity + 3 * g;
             Q4 q = {R[0], R[1], R[2], R[3]};
-            hw.body.push_back(make_body(h, v3(P[0], P[1], P[2]), q, v3(LV[0], LV[1], LV[2]), v3(AV[0], AV[1], AV[2]),
-                                        d->body_mass[g], d->body_fixed[g] != 0, d->body_material + 3 * g,
-                                        hw.scene_err[s]));
-            hw.body_hull.push_back(it->second);
+            hw.body.push_back(make_body(hw, c0, (int)nc, v3(P[0], P[1], P[2]), q, v3(LV[0], LV[1], LV[2]),
+                                        v3(AV[0], AV[1], AV[2]), d->body_mass[g], d->body_fixed[g] != 0,
+                                        d->body_material + 3 * g, hw.scene_err[s]));
+            hw.body_hull.push_back(hw.cols[c0].type == COL_HULL ? hw.cols[c0].hull : -1);
         }
         hw.scene_body_off.push_back((int32_t)hw.body.size());
         if (d->scene_constraint_offset && d->constraints) {
@@ -911,8 +1002,8 @@ void rp_world_hull(rp_world* w, uint32_t body, uint32_t counts[6], double* verti
                    uint32_t* face_off, uint32_t* face_el, uint32_t* vf_off, uint32_t* vf, uint32_t* vn_off,
                    uint32_t* vn, uint32_t* ff_off, uint32_t* ff) {
     memset(counts, 0, 6 * sizeof(uint32_t));
-    if (body >= w->hw.body.size()) return;
-    const HostHull& h = w->hw.hulls[w->hw.body_hull[body]];
+    if (body >= w->hw.body.size() || w->hw.body_hull[body] < 0) return;
+    const HostHull& h = w->hw.hulls[w->hw.body_hull[body]];  // the body's first collider
     counts[0] = (uint32_t)h.v.size(); counts[1] = (uint32_t)h.fn.size();
     counts[2] = (uint32_t)h.fe.size(); counts[3] = (uint32_t)h.vf.size();
     counts[4] = (uint32_t)h.vn.size(); counts[5] = (uint32_t)h.ff.size();
@@ -951,9 +1042,9 @@ uint32_t rp_world_set_body_params(rp_world* w, const double* mass, const double*
         if (mu_d) e.mu_d = h.mu_d = mu_d[b];
         if (mass && !e.fixed) {
             // as at creation (make_body): 1/m, the hull's inertia and its inverse
-            const HostHull& hull = w->hw.hulls[w->hw.body_hull[b]];
+            int c0 = w->hw.body_col_off[b], nc = w->hw.body_col_off[b + 1] - c0;
             e.inv_mass = h.inv_mass = 1.0 / mass[b];
-            e.I = h.I = hull_inertia(hull, mass[b]);
+            e.I = h.I = colliders_inertia(w->hw.hulls, w->hw.cols.data() + c0, nc, mass[b], err);
             if (!m3_inverse(e.I, e.Iinv)) err |= ERR_HULL;
             h.Iinv = e.Iinv;
         }
@@ -1151,30 +1242,49 @@ static bool dropin_build(rp_entity** ents, size_t n, rp_constraint* cons, HostWo
     for (size_t i = 0; i < n; ++i) id2idx[ents[i]->id] = (int32_t)i;
     for (size_t i = 0; i < n; ++i) {
         const rp_entity* e = ents[i];
-        if (la_len(e->colliders) != 1 || e->colliders[0].type != RP_COLLIDER_CONVEX_HULL) {
-            set_msg("drop-in supports entities with exactly one convex-hull collider");
+        size_t ncol = la_len(e->colliders);
+        if (ncol == 0) {
+            set_msg("an entity has no collider");
             return false;
         }
-        const rp_convex_hull& ch = e->colliders[0].u.convex_hull;
-        HostHull h;
-        size_t nv = la_len(ch.vertices), nf = la_len(ch.faces);
-        for (size_t k = 0; k < nv; ++k) h.v.push_back(to_v3(ch.vertices[k]));
-        std::vector<std::vector<int32_t>> fe(nf), vf(nv), vn(nv), ff(nf);
-        for (size_t f = 0; f < nf; ++f) {
-            h.fn.push_back(to_v3(ch.faces[f].normal));
-            for (size_t k = 0; k < la_len(ch.faces[f].elements); ++k) fe[f].push_back((int32_t)ch.faces[f].elements[k]);
-            for (size_t k = 0; k < la_len(ch.face_to_neighbors[f]); ++k) ff[f].push_back((int32_t)ch.face_to_neighbors[f][k]);
+        int32_t first_hull = -1;
+        for (size_t ci = 0; ci < ncol; ++ci) {
+            const rp_collider& col = e->colliders[ci];
+            if (col.type == RP_COLLIDER_SPHERE) {
+                hw.cols.push_back(HostCollider{COL_SPHERE, -1, col.u.sphere.radius});
+                continue;
+            }
+            if (col.type != RP_COLLIDER_CONVEX_HULL) {
+                set_msg("unknown collider type");
+                return false;
+            }
+            // the reference's own hull topology (collider.cpp:194-364)
+            const rp_convex_hull& ch = col.u.convex_hull;
+            HostHull h;
+            size_t nv = la_len(ch.vertices), nf = la_len(ch.faces);
+            for (size_t k = 0; k < nv; ++k) h.v.push_back(to_v3(ch.vertices[k]));
+            std::vector<std::vector<int32_t>> fe(nf), vf(nv), vn(nv), ff(nf);
+            for (size_t f = 0; f < nf; ++f) {
+                h.fn.push_back(to_v3(ch.faces[f].normal));
+                for (size_t k = 0; k < la_len(ch.faces[f].elements); ++k) fe[f].push_back((int32_t)ch.faces[f].elements[k]);
+                for (size_t k = 0; k < la_len(ch.face_to_neighbors[f]); ++k)
+                    ff[f].push_back((int32_t)ch.face_to_neighbors[f][k]);
+            }
+            for (size_t v = 0; v < nv; ++v) {
+                for (size_t k = 0; k < la_len(ch.vertex_to_faces[v]); ++k) vf[v].push_back((int32_t)ch.vertex_to_faces[v][k]);
+                for (size_t k = 0; k < la_len(ch.vertex_to_neighbors[v]); ++k)
+                    vn[v].push_back((int32_t)ch.vertex_to_neighbors[v][k]);
+            }
+            flatten(fe, h.fe_off, h.fe);
+            flatten(vf, h.vf_off, h.vf);
+            flatten(vn, h.vn_off, h.vn);
+            flatten(ff, h.ff_off, h.ff);
+            hw.hulls.push_back(h);
+            hw.cols.push_back(HostCollider{COL_HULL, (int32_t)hw.hulls.size() - 1, 0.0f});
+            if (ci == 0) first_hull = (int32_t)hw.hulls.size() - 1;
         }
-        for (size_t v = 0; v < nv; ++v) {
-            for (size_t k = 0; k < la_len(ch.vertex_to_faces[v]); ++k) vf[v].push_back((int32_t)ch.vertex_to_faces[v][k]);
-            for (size_t k = 0; k < la_len(ch.vertex_to_neighbors[v]); ++k) vn[v].push_back((int32_t)ch.vertex_to_neighbors[v][k]);
-        }
-        flatten(fe, h.fe_off, h.fe);
-        flatten(vf, h.vf_off, h.vf);
-        flatten(vn, h.vn_off, h.vn);
-        flatten(ff, h.ff_off, h.ff);
-        hw.hulls.push_back(h);
-        hw.body_hull.push_back((int32_t)i);
+        hw.body_col_off.push_back((int32_t)hw.cols.size());
+        hw.body_hull.push_back(first_hull);
         Body b;
         memset(&b, 0, sizeof(b));
         hw.body.push_back(b);
@@ -1268,7 +1378,13 @@ extern "C" uint32_t rp_pbd_simulate(double dt, rp_entity** ents, rp_constraint* 
     for (size_t i = 0; i < n; ++i) {
         key.push_back(ents[i]);
         key.push_back(ents[i]->colliders);
-        key.push_back(ents[i]->colliders ? ents[i]->colliders[0].u.convex_hull.vertices : nullptr);
+        for (size_t ci = 0; ci < la_len(ents[i]->colliders); ++ci) {
+            const rp_collider& c = ents[i]->colliders[ci];
+            uint32_t rbits = 0;
+            if (c.type != RP_COLLIDER_CONVEX_HULL) memcpy(&rbits, &c.u.sphere.radius, 4);
+            key.push_back(c.type == RP_COLLIDER_CONVEX_HULL ? (const void*)c.u.convex_hull.vertices
+                                                            : (const void*)(((uintptr_t)rbits << 8) | 0xA5u));
+        }
     }
     key.push_back((const void*)(uintptr_t)la_len(cons));
     for (size_t k = 0; k < la_len(cons); ++k) {

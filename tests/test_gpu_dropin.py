@@ -63,3 +63,10 @@ def test_dropin_dt_nonpositive_is_noop():
     lib = load("rp")
     assert lib.rp_pbd_simulate(0.0, None, None, 20, 1, 1) == 0
     assert lib.rp_pbd_simulate(-1.0, None, None, 20, 1, 1) == 0
+
+
+def test_dropin_spheres_and_compounds_bit_exact():
+    """Entities whose colliders the reference built (collider_sphere_create,
+    several convex hulls per entity) through the drop-in."""
+    sa, sb = _run_pair(scenes.batched_mixed(1, first_seed=7), 120)
+    assert not state_bits_equal(sa, sb)

@@ -195,3 +195,13 @@ def test_graph_replays_match_eager_launches():
     o.step(3, substeps=10)
     gg.step(3, substeps=10)
     assert not state_bits_equal(o.state(), gg.state())
+
+
+def test_spheres_and_compounds_bit_exact():
+    """Sphere colliders and two-hull compound bodies (general narrow path:
+    one lane per pair over every collider pair) match the oracle bit for bit;
+    the oracle matches the compiled reference on these scenes
+    (tests/golden/traj_mixed.npz)."""
+    world = scenes.batched_mixed(8, first_seed=40)
+    o, g = _pair(world, frames=150, check_every=30)
+    assert g.lib.rp_world_error(g.h, 0) == 0
