@@ -139,8 +139,13 @@ typedef struct {
 /* pbd_simulate_with_constraints (src/physics/pbd.h:94, pbd.cpp:468-747) with C
  * linkage. `entities` is the reference's light_array of Entity*, and
  * `external_constraints` a light_array of Constraint or NULL. Entities are
- * updated in place. Returns the error word (0 on success): bits of
- * RP_ERR_* where the reference would abort. dt <= 0 is a no-op. */
+ * updated in place, including the reference's side effect on their
+ * colliders: with collisions on, every entity of a pair the collision pass
+ * visits has its hulls' transformed_vertices / transformed_faces normals and
+ * its spheres' centers set for its pose after the last substep's
+ * integration (pbd.cpp:598-599, collider.cpp:409-438). Returns the error
+ * word (0 on success): bits of RP_ERR_* where the reference would abort.
+ * dt <= 0 is a no-op. */
 uint32_t rp_pbd_simulate(double dt, rp_entity** entities, rp_constraint* external_constraints,
                          uint32_t num_substeps, uint32_t num_pos_iters, int32_t enable_collisions);
 
@@ -231,6 +236,7 @@ enum {
     RP_ERR_SOLVER_CAPACITY = 1u << 11,  /* constraint list exceeded its capacity */
     RP_ERR_PAIR_CAPACITY = 1u << 12,    /* broad-phase pairs of a large scene exceeded its capacity */
     RP_ERR_ADJOINT_SCOPE = 1u << 13,    /* adjoint on a scene outside its scope (one dynamic body) */
+    RP_ERR_OBJ = 1u << 14,              /* OBJ file unreadable or malformed (rp_obj_parse) */
     RP_ERR_RUNTIME = 1u << 31         /* HIP runtime failure (see rp_last_error_string) */
 };
 
@@ -264,6 +270,14 @@ uint64_t rp_world_epa_failures(rp_world* w);
 uint64_t rp_world_gjk_cycle_exits(rp_world* w);
 /* The HIP stream the world's kernels run on (a hipStream_t). */
 void* rp_world_stream(rp_world* w);
+/* Every body's model matrix as entity_get_model_matrix computes it
+ * (src/entity.cpp:123-145: translation * rotation * scale, with the body's
+ * world scale), row-major [nb][4][4]: doubles, or with as_float the float
+ * array util_matrix_to_r32_array (src/util.cpp:77-94) makes for the renderer.
+ * out_on_device: out is device memory and the kernel is queued on the
+ * world's stream without waiting; otherwise out is host memory and the call
+ * blocks. Returns 0 or RP_ERR_RUNTIME. */
+uint32_t rp_world_model_matrices(rp_world* w, void* out, int32_t as_float, int32_t out_on_device);
 
 /* ---- Gradients (reverse mode over a recorded run) --------------------------
  * Scope: every scene has one dynamic body (any number of fixed bodies), no
@@ -336,6 +350,31 @@ void rp_world_hull(rp_world* w, uint32_t body, uint32_t counts[6], double* verti
  * the reference only if the device sqrt and division are correctly rounded;
  * this entry point lets a test check that on the installed device. */
 uint32_t rp_fp64_selftest(const double* a, const double* b, double* out, uint64_t n);
+
+/* ---- OBJ assets --------------------------------------------------------------
+ * obj_parse (src/render/obj.h:7, obj.cpp:7-151): the triangles of a Wavefront
+ * OBJ file as one vertex per triangle corner, in face order, with the values
+ * the reference's bundled tinyobjloader produces (its own decimal parser,
+ * then float; polygons ear-clipped as it does). Normals come from the file
+ * or, when it has none, are each triangle's unnormalised cross product.
+ * obj_parse's index array is 0, 1, 2, ... and is not returned. The hull of a
+ * shape is built from the corner positions (rp_world_desc.shape_corners, or
+ * collider_convex_hull_create in the reference). */
+#pragma pack(push, 1)
+typedef struct {
+    float position[3];
+    float normal[3];
+    float texture_coordinates[2]; /* 0 when the file has none */
+} rp_vertex;                      /* Vertex (src/render/mesh.h:5-10), 32 B */
+#pragma pack(pop)
+/* Writes min(capacity, n) corners and sets *num_vertices = n (call with
+ * vertices = NULL to size). Returns 0, or RP_ERR_OBJ where obj_parse would
+ * exit (unreadable file, index 0) or read out of bounds (an index past the
+ * file's arrays); rp_last_error_string says which. */
+uint32_t rp_obj_parse(const char* obj_path, rp_vertex* vertices, uint32_t capacity, uint32_t* num_vertices);
+/* The same over OBJ text in memory. */
+uint32_t rp_obj_parse_memory(const char* text, size_t length, rp_vertex* vertices, uint32_t capacity,
+                             uint32_t* num_vertices);
 
 /* Last error of the calling thread: the error word of the last drop-in call
  * and a message for runtime failures. */

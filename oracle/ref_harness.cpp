@@ -20,6 +20,7 @@
 #include "physics/collider.h"
 #include "physics/broad.h"
 #include "examples/examples_util.h"
+#include "render/obj.h"
 #include "entity.h"
 #include "rp_oracle.h"
 
@@ -327,4 +328,51 @@ API uint32_t ref_world_step_with(void* wp, void* fn, double dt, uint32_t frames,
         }
     }
     return err;
+}
+
+/* The transformed geometry entity g's colliders hold (what collider_update,
+ * collider.cpp:409-438, last wrote): per collider in order, a hull's
+ * transformed vertices then its transformed face normals, or a sphere's
+ * center, as doubles. Returns the number of doubles (written up to cap). */
+API uint32_t ref_world_collider_geometry(void* wp, uint32_t g, double* out, uint32_t cap) {
+    Entity* e = body_at((ref_world*)wp, g);
+    uint32_t n = 0;
+    auto put = [&](vec3 v) {
+        if (n + 3 <= cap) memcpy(out + n, &v, 24);
+        n += 3;
+    };
+    for (u32 ci = 0; ci < array_length(e->colliders); ++ci) {
+        Collider* c = &e->colliders[ci];
+        if (c->type == COLLIDER_TYPE_SPHERE) {
+            put(c->sphere.center);
+            continue;
+        }
+        for (u32 k = 0; k < array_length(c->convex_hull.transformed_vertices); ++k)
+            put(c->convex_hull.transformed_vertices[k]);
+        for (u32 k = 0; k < array_length(c->convex_hull.transformed_faces); ++k)
+            put(c->convex_hull.transformed_faces[k].normal);
+    }
+    return n;
+}
+
+/* entity_get_model_matrix (entity.cpp:123-145) of entity g, row-major. */
+API void ref_world_model_matrix(void* wp, uint32_t g, double out[16]) {
+    mat4 m = entity_get_model_matrix(body_at((ref_world*)wp, g));
+    memcpy(out, m.data, 128);
+}
+
+/* obj_parse (src/render/obj.cpp:7-151) of a file: copies min(cap, n) Vertex
+ * records (32 bytes each) to out and returns n. obj_parse ends the process on
+ * a file it cannot load, so only call this with valid files. */
+API uint32_t ref_obj_parse(const char* path, void* out, uint32_t cap) {
+    Vertex* verts = NULL;
+    u32* idx = NULL;
+    obj_parse(path, &verts, &idx);
+    uint32_t n = array_length(verts);
+    for (uint32_t k = 0; k < n; ++k)
+        if (idx[k] != k) return UINT32_MAX; /* the index array is the identity */
+    if (out) memcpy(out, verts, sizeof(Vertex) * (n < cap ? n : cap));
+    array_free(verts);
+    array_free(idx);
+    return n;
 }

@@ -43,6 +43,9 @@ def library():
         "rp_world_epa_failures": (u64, [_P]),
         "rp_world_gjk_cycle_exits": (u64, [_P]),
         "rp_world_stream": (_P, [_P]),
+        "rp_world_model_matrices": (u32, [_P, _P, i32, i32]),
+        "rp_obj_parse": (u32, [ctypes.c_char_p, _P, u32, ctypes.POINTER(u32)]),
+        "rp_obj_parse_memory": (u32, [ctypes.c_char_p, ctypes.c_size_t, _P, u32, ctypes.POINTER(u32)]),
         "rp_world_body_info": (None, [_P, _P, _P, _P, _P]),
         "rp_world_broad_pairs": (u32, [_P, u32, _P, u32]),
         "rp_world_contacts": (u32, [_P, u32, u32, u32, _P, u32]),
@@ -76,6 +79,34 @@ def library():
 
 def last_error_string() -> str:
     return (library().rp_last_error_string() or b"").decode()
+
+
+def load_obj(path=None, text=None) -> np.ndarray:
+    """obj_parse (src/render/obj.cpp:7-151) of an OBJ file or of OBJ text: one
+    record per triangle corner with fields position (3 floats), normal (3)
+    and texture_coordinates (2). `rec["position"]` is the corner stream a
+    World shape takes."""
+    lib = library()
+    dt = np.dtype([("position", np.float32, 3), ("normal", np.float32, 3), ("texture_coordinates", np.float32, 2)])
+    n = ctypes.c_uint32()
+
+    def call(buf, cap):
+        if text is not None:
+            raw = text.encode() if isinstance(text, str) else bytes(text)
+            return lib.rp_obj_parse_memory(raw, len(raw), buf, cap, ctypes.byref(n))
+        return lib.rp_obj_parse(str(path).encode(), buf, cap, ctypes.byref(n))
+
+    if (path is None) == (text is None):
+        raise ValueError("give exactly one of path and text")
+    err = call(None, 0)
+    if err:
+        raise RuntimeError(f"obj_parse failed ({err:#x}): {last_error_string()}")
+    out = np.zeros(n.value, dtype=dt)
+    if n.value:
+        err = call(out.ctypes.data, n.value)
+        if err:
+            raise RuntimeError(f"obj_parse failed ({err:#x}): {last_error_string()}")
+    return out
 
 
 class World:
@@ -157,6 +188,14 @@ class World:
 
     def gjk_cycle_exits(self) -> int:
         return int(self.lib.rp_world_gjk_cycle_exits(self.h))
+
+    def model_matrices(self, as_float: bool = False) -> np.ndarray:
+        """entity_get_model_matrix of every body: (nb, 4, 4), float64 or float32."""
+        out = np.zeros((self.nb, 4, 4), dtype=np.float32 if as_float else np.float64)
+        err = self.lib.rp_world_model_matrices(self.h, out.ctypes.data, 1 if as_float else 0, 0)
+        if err:
+            raise RuntimeError(f"model_matrices failed ({err:#x}): {last_error_string()}")
+        return out
 
     KERNEL_CLASSES = ("frame", "integrate", "transform", "narrow", "solve", "manifold")
 

@@ -17,7 +17,7 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 LIB = HERE / "librawphys.so"
 SOURCES = [HERE / "csrc" / "rp_kernels.hip", HERE / "csrc" / "rp_large.hip", HERE / "csrc" / "rp_adjoint.hip",
-           HERE / "csrc" / "rp_world.cpp", HERE / "csrc" / "rp_refsyms.cpp"]
+           HERE / "csrc" / "rp_world.cpp", HERE / "csrc" / "rp_refsyms.cpp", HERE / "csrc" / "rp_obj.cpp"]
 HEADERS = [HERE / "csrc" / "rp_kernels.h", HERE / "csrc" / "rp_device.h", HERE / "csrc" / "rp_dual.h",
            HERE / "csrc" / "rp_step.h",
            HERE / "csrc" / "rp_math.h",

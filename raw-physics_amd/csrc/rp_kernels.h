@@ -25,6 +25,8 @@ struct DevWorld {
     // bodies
     Body* body;                     // [nb]
     M4* model;                      // [nb]  model matrix of the current substep
+    double* pose;                   // [nb][7] (x, q) after the last substep's integration (FrameParams::keep_pose)
+    const V3* body_scale;           // [nb]  world scale (rendering model matrices)
     const int32_t* body_hull;       // [nb]
     const int32_t* body_tv_off;     // [nb]
     // hull instances (vertices scaled, local frame)
@@ -117,6 +119,7 @@ struct FrameParams {
     unsigned substeps = 0, iters = 0;
     bool collisions = true;
     bool forces_given = false;  // body F/T already hold the frame's force sums
+    bool keep_pose = false;     // record every body's pose after the last substep's integration
 };
 
 // Kernel classes timed by the profiling mode.
@@ -168,6 +171,9 @@ int scan_block_count(int64_t n);
 hipError_t launch_pair_contacts(const DevWorld& w, int b1, int b2, double* out, int* count, uint32_t* err,
                                 hipStream_t st);
 hipError_t launch_broad(const DevWorld& w, hipStream_t st, const BigScene* bigs = nullptr, int nbig = 0);
+// entity_get_model_matrix of every body into out ([nb][4][4] row-major,
+// double or float), device memory.
+hipError_t launch_model_matrices(const DevWorld& w, void* out, bool f32, hipStream_t st);
 // Lanes per pair the narrow phase can use for this world (LDS permitting).
 bool narrow_group_supported(const DevWorld& w, int g);
 int narrow_blocks_for(const DevWorld& w, int g, int64_t pair_capacity);

@@ -189,15 +189,32 @@ __global__ void k_integrate(DevWorld w, double h) {
     w.model[b] = model_matrix(e.x, e.q);
 }
 
-// collider_update for one vertex (collider.cpp:413-421): gm_mat4_multiply_vec4
-// of M with [v, 1] (gm.h:405-413), then scaled by 1/w.
-__device__ __forceinline__ V3 transform_vertex(const M4& M, V3 v) {
-    double x = v.x * M.m[0][0] + v.y * M.m[0][1] + v.z * M.m[0][2] + 1.0 * M.m[0][3];
-    double y = v.x * M.m[1][0] + v.y * M.m[1][1] + v.z * M.m[1][2] + 1.0 * M.m[1][3];
-    double z = v.x * M.m[2][0] + v.y * M.m[2][1] + v.z * M.m[2][2] + 1.0 * M.m[2][3];
-    double ww = v.x * M.m[3][0] + v.y * M.m[3][1] + v.z * M.m[3][2] + 1.0 * M.m[3][3];
-    double sc = 1.0 / ww;
-    return v3(sc * x, sc * y, sc * z);
+// The pose every body has when the substep's collision pass reads it
+// (colliders_update at pbd.cpp:598-599 runs after integration): launched
+// after the last substep's integration when the caller wants the colliders'
+// transformed geometry of that pose written back.
+__global__ void k_keep_pose(DevWorld w) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= w.nb) return;
+    const Body& e = w.body[b];
+    double* p = w.pose + 7 * (size_t)b;
+    p[0] = e.x.x; p[1] = e.x.y; p[2] = e.x.z;
+    p[3] = e.q.x; p[4] = e.q.y; p[5] = e.q.z; p[6] = e.q.w;
+}
+
+// entity_get_model_matrix of every body, row-major 4x4, as double or as the
+// float array util_matrix_to_r32_array (util.cpp:77-94) hands to the renderer.
+template <class F>
+__global__ void k_model_out(DevWorld w, F* out) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= w.nb) return;
+    const Body& e = w.body[b];
+    M4 M = entity_model_matrix(e.x, e.q, w.body_scale[b]);
+    F* o = out + 16 * (size_t)b;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) o[4 * i + j] = (F)M.m[i][j];
 }
 
 // Transformed vertices of the bodies that take part in a live pair this
@@ -633,6 +650,8 @@ hipError_t launch_frame(const DevWorld& w, const FrameParams& fp, hipStream_t st
         t.begin(KC_INTEGRATE, st);
         hipLaunchKernelGGL(k_integrate, dim3(cdiv(w.nb, 128)), dim3(128), 0, st, w, fp.h);
         t.end(st);
+        if (fp.keep_pose && sub + 1 == fp.substeps)
+            hipLaunchKernelGGL(k_keep_pose, dim3(cdiv(w.nb, 256)), dim3(256), 0, st, w);
         if (fp.collisions) {
             if (w.narrow_group == 1 || w.manifold_group == 1 || w.has_general) {
                 t.begin(KC_TRANSFORM, st);
@@ -676,6 +695,13 @@ hipError_t launch_pair_contacts(const DevWorld& w, int b1, int b2, double* out, 
     hipLaunchKernelGGL(k_model, dim3(cdiv(w.nb, 128)), dim3(128), 0, st, w);
     hipLaunchKernelGGL(k_transform, dim3(cdiv(w.n_xf, 256)), dim3(256), 0, st, w, 1);
     hipLaunchKernelGGL(k_pair_contacts, dim3(1), dim3(1), 0, st, w, b1, b2, out, count, err);
+    return hipGetLastError();
+}
+
+hipError_t launch_model_matrices(const DevWorld& w, void* out, bool f32, hipStream_t st) {
+    if (w.nb == 0) return hipSuccess;
+    if (f32) hipLaunchKernelGGL(k_model_out<float>, dim3(cdiv(w.nb, 256)), dim3(256), 0, st, w, (float*)out);
+    else hipLaunchKernelGGL(k_model_out<double>, dim3(cdiv(w.nb, 256)), dim3(256), 0, st, w, (double*)out);
     return hipGetLastError();
 }
 

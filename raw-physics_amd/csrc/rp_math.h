@@ -140,6 +140,40 @@ RP_HD M4 model_matrix(V3 t, Q4 q) {
     M4 T = {{{1.0, 0.0, 0.0, t.x}, {0.0, 1.0, 0.0, t.y}, {0.0, 0.0, 1.0, t.z}, {0.0, 0.0, 0.0, 1.0}}};
     return m4_mul(T, R);
 }
+// entity_get_model_matrix (entity.cpp:123-145): T * (R * S), the matrix a
+// renderer draws the body's mesh with.
+RP_HD M4 entity_model_matrix(V3 t, Q4 q, V3 s) {
+    M3 r3 = q_mat3(q);
+    M4 Rq;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) Rq.m[i][j] = r3.m[i][j];
+        Rq.m[i][3] = 0.0;
+        Rq.m[3][i] = 0.0;
+    }
+    Rq.m[3][3] = 1.0;
+    M4 S = {{{s.x, 0.0, 0.0, 0.0}, {0.0, s.y, 0.0, 0.0}, {0.0, 0.0, s.z, 0.0}, {0.0, 0.0, 0.0, 1.0}}};
+    M4 T = {{{1.0, 0.0, 0.0, t.x}, {0.0, 1.0, 0.0, t.y}, {0.0, 0.0, 1.0, t.z}, {0.0, 0.0, 0.0, 1.0}}};
+    return m4_mul(T, m4_mul(Rq, S));
+}
+// collider_update for one hull vertex (collider.cpp:413-422): gm_mat4_multiply_vec4
+// of M with [v, 1] (gm.h:405-413), then scaled by 1/w.
+RP_HD V3 transform_vertex(const M4& M, V3 v) {
+    double x = v.x * M.m[0][0] + v.y * M.m[0][1] + v.z * M.m[0][2] + 1.0 * M.m[0][3];
+    double y = v.x * M.m[1][0] + v.y * M.m[1][1] + v.z * M.m[1][2] + 1.0 * M.m[1][3];
+    double z = v.x * M.m[2][0] + v.y * M.m[2][1] + v.z * M.m[2][2] + 1.0 * M.m[2][3];
+    double ww = v.x * M.m[3][0] + v.y * M.m[3][1] + v.z * M.m[3][2] + 1.0 * M.m[3][3];
+    double sc = 1.0 / ww;
+    return v3(sc * x, sc * y, sc * z);
+}
+// collider_update for one face normal (collider.cpp:425-429):
+// gm_mat4_multiply_vec3(M, n, false) (gm.h:448-460), then normalized.
+RP_HD V3 transform_normal(const M4& M, V3 n) {
+    V3 r = v3(M.m[0][0] * n.x + M.m[0][1] * n.y + M.m[0][2] * n.z, M.m[1][0] * n.x + M.m[1][1] * n.y + M.m[1][2] * n.z,
+              M.m[2][0] * n.x + M.m[2][1] * n.y + M.m[2][2] * n.z);
+    return nrm(r);
+}
 // quaternion.cpp:3-16
 RP_HD Q4 q_from_axis_radians(V3 axis, double angle) {
     if (len(axis) != 0.0) axis = nrm(axis);

@@ -20,6 +20,11 @@ static thread_local uint32_t g_last_err = 0;
 static thread_local std::string g_last_msg;
 
 static void set_msg(const std::string& m) { g_last_msg = m; }
+// Error word and message of a failed call made outside this file (rp_obj.cpp).
+void rp_set_last_error(uint32_t code, const std::string& msg) {
+    g_last_err = code;
+    g_last_msg = msg;
+}
 
 #define RP_CHECK(expr)                                                                    \
     do {                                                                                  \
@@ -305,6 +310,7 @@ struct HostWorld {
     std::vector<int32_t> scene_body_off{0}, scene_ext_off{0};
     std::vector<Body> body;
     std::vector<int32_t> body_hull;     // first collider's hull instance (-1: a sphere)
+    std::vector<V3> body_scale;         // world scale (rendering only)
     std::vector<int32_t> body_col_off{0};
     std::vector<HostCollider> cols;     // every body's colliders, in order
     std::vector<HostHull> hulls;
@@ -487,6 +493,10 @@ static bool build_image(const HostWorld& hw, DeviceImage& im) {
     ok &= im.upload(d.scene_item_base, con_base);
     ok &= im.upload_mut(d.body, hw.body);
     ok &= im.alloc(d.model, d.nb);
+    ok &= im.alloc(d.pose, (size_t)d.nb * 7);
+    std::vector<V3> scale = hw.body_scale;
+    scale.resize(d.nb, v3(1.0, 1.0, 1.0));
+    ok &= im.upload(d.body_scale, scale);
     ok &= im.upload(d.body_hull, hw.body_hull);
     const int32_t* p;
     ok &= im.upload(p, im.body_tv_off); d.body_tv_off = p;
@@ -639,6 +649,7 @@ static bool host_world_from_desc(const rp_world_desc* d, HostWorld& hw) {
                                         v3(AV[0], AV[1], AV[2]), d->body_mass[g], d->body_fixed[g] != 0,
                                         d->body_material + 3 * g, hw.scene_err[s]));
             hw.body_hull.push_back(hw.cols[c0].type == COL_HULL ? hw.cols[c0].hull : -1);
+            hw.body_scale.push_back(v3(sc[0], sc[1], sc[2]));
         }
         hw.scene_body_off.push_back((int32_t)hw.body.size());
         if (d->scene_constraint_offset && d->constraints) {
@@ -715,7 +726,7 @@ constexpr uint32_t kFramesPerChunk = 32;
 
 static bool same_params(const FrameParams& a, const FrameParams& b) {
     return a.dt == b.dt && a.h == b.h && a.gravity == b.gravity && a.substeps == b.substeps && a.iters == b.iters &&
-           a.collisions == b.collisions && a.forces_given == b.forces_given;
+           a.collisions == b.collisions && a.forces_given == b.forces_given && a.keep_pose == b.keep_pose;
 }
 
 // The graph of one frame for these parameters and the current schedules,
@@ -940,6 +951,26 @@ static uint64_t read_stat(rp_world* w, int k) {
 }
 uint64_t rp_world_epa_failures(rp_world* w) { return read_stat(w, STAT_EPA_FAIL); }
 uint64_t rp_world_gjk_cycle_exits(rp_world* w) { return read_stat(w, STAT_GJK_CYCLE); }
+
+uint32_t rp_world_model_matrices(rp_world* w, void* out, int32_t as_float, int32_t out_on_device) {
+    const DevWorld& d = w->im.d;
+    if (!out || d.nb == 0) return 0;
+    size_t bytes = (size_t)d.nb * 16 * (as_float ? sizeof(float) : sizeof(double));
+    if (out_on_device) {
+        if (launch_model_matrices(d, out, as_float != 0, w->stream) != hipSuccess) return RP_ERR_RUNTIME;
+        return 0;
+    }
+    void* tmp = nullptr;
+    uint32_t err = 0;
+    if (hipMalloc(&tmp, bytes) != hipSuccess || launch_model_matrices(d, tmp, as_float != 0, w->stream) != hipSuccess ||
+        hipMemcpyAsync(out, tmp, bytes, hipMemcpyDeviceToHost, w->stream) != hipSuccess ||
+        hipStreamSynchronize(w->stream) != hipSuccess) {
+        set_msg("model matrices: device error");
+        err = RP_ERR_RUNTIME;
+    }
+    if (tmp) (void)hipFree(tmp);
+    return err;
+}
 
 void rp_world_body_info(rp_world* w, double* inv_mass, double* inertia, double* inv_inertia, double* radius) {
     for (size_t i = 0; i < w->hw.body.size(); ++i) {
@@ -1285,6 +1316,7 @@ static bool dropin_build(rp_entity** ents, size_t n, rp_constraint* cons, HostWo
         }
         hw.body_col_off.push_back((int32_t)hw.cols.size());
         hw.body_hull.push_back(first_hull);
+        hw.body_scale.push_back(to_v3(e->world_scale));
         Body b;
         memset(&b, 0, sizeof(b));
         hw.body.push_back(b);
@@ -1368,6 +1400,40 @@ static Body entity_body(const rp_entity* e) {
     return b;
 }
 
+// The collision pass's side effect on the entities' colliders (pbd.cpp:598-599
+// -> collider_update, collider.cpp:409-438): both entities of every pair the
+// pass visits get their colliders transformed to the pose they have after the
+// substep's integration. The visited set is fixed for the call (activity does
+// not change within it), so after the call each visited entity's colliders
+// hold its pose after the last substep's integration, which the device kept;
+// the others are left as they were.
+static bool dropin_write_colliders(rp_world* w, rp_entity** ents, size_t n) {
+    std::vector<int32_t> visited(n);
+    std::vector<double> pose(7 * n);
+    RP_CHECK(hipMemcpy(visited.data(), w->im.d.need, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    RP_CHECK(hipMemcpy(pose.data(), w->im.d.pose, n * 7 * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i) {
+        if (!visited[i]) continue;
+        const double* p = pose.data() + 7 * i;
+        V3 x = v3(p[0], p[1], p[2]);
+        Q4 q = {p[3], p[4], p[5], p[6]};
+        M4 M = model_matrix(x, q);  // util_get_model_matrix_no_scale (util.cpp:60-74)
+        rp_collider* cols = ents[i]->colliders;
+        for (size_t ci = 0; ci < la_len(cols); ++ci) {
+            if (cols[ci].type == RP_COLLIDER_SPHERE) {
+                cols[ci].u.sphere.center = from_v3(x);
+                continue;
+            }
+            rp_convex_hull& ch = cols[ci].u.convex_hull;
+            for (size_t k = 0; k < la_len(ch.transformed_vertices); ++k)
+                ch.transformed_vertices[k] = from_v3(transform_vertex(M, to_v3(ch.vertices[k])));
+            for (size_t k = 0; k < la_len(ch.transformed_faces); ++k)
+                ch.transformed_faces[k].normal = from_v3(transform_normal(M, to_v3(ch.faces[k].normal)));
+        }
+    }
+    return true;
+}
+
 extern "C" uint32_t rp_pbd_simulate(double dt, rp_entity** ents, rp_constraint* cons, uint32_t substeps,
                                     uint32_t iters, int32_t collisions) {
     g_last_err = 0;
@@ -1425,8 +1491,10 @@ extern "C" uint32_t rp_pbd_simulate(double dt, rp_entity** ents, rp_constraint* 
     fp.iters = iters;
     fp.collisions = collisions != 0;
     fp.forces_given = true;
+    fp.keep_pose = fp.collisions && substeps > 0;
     if (world_run(w, fp, 1, true)) return g_last_err = RP_ERR_RUNTIME;
     if (!download_bodies(w)) return g_last_err = RP_ERR_RUNTIME;
+    if (fp.keep_pose && !dropin_write_colliders(w, ents, n)) return g_last_err = RP_ERR_RUNTIME;
     for (size_t i = 0; i < n; ++i) {
         rp_entity* e = ents[i];
         const Body& b = w->host_bodies[i];

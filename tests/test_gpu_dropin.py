@@ -24,7 +24,7 @@ def _dropin_fn():
     return ctypes.cast(lib.rp_pbd_simulate, ctypes.c_void_p).value
 
 
-def _run_pair(world, frames, **kw):
+def _run_pair(world, frames, worlds=False, **kw):
     ref = load("ref")
     ref.ref_world_step_with.restype = ctypes.c_uint32
     ref.ref_world_step_with.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_uint32,
@@ -38,7 +38,23 @@ def _run_pair(world, frames, **kw):
         a.step(1, dt=dt, substeps=sub, iters=it, collisions=bool(coll))
         err = ref.ref_world_step_with(b.h, fn, dt, 1, sub, it, coll, 10.0)
         assert err == 0, f"frame {f}: error word {err:#x}"
+    if worlds:
+        return a, b
     return a.state(), b.state()
+
+
+def _geometry(sim):
+    """Every body's collider geometry as the entities hold it."""
+    ref = load("ref")
+    ref.ref_world_collider_geometry.restype = ctypes.c_uint32
+    ref.ref_world_collider_geometry.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32]
+    out = []
+    for g in range(sim.world.num_bodies):
+        n = ref.ref_world_collider_geometry(sim.h, g, None, 0)
+        buf = np.zeros(n)
+        ref.ref_world_collider_geometry(sim.h, g, buf.ctypes.data, n)
+        out.append(buf)
+    return out
 
 
 def test_dropin_stack_bit_exact():
@@ -70,3 +86,67 @@ def test_dropin_spheres_and_compounds_bit_exact():
     several convex hulls per entity) through the drop-in."""
     sa, sb = _run_pair(scenes.batched_mixed(1, first_seed=7), 120)
     assert not state_bits_equal(sa, sb)
+
+
+def _writeback_scene():
+    """Floor; a cube resting on it (asleep after about 75 frames); a sphere and
+    a two-box compound falling onto it; a cube far above everything (in no
+    broad pair, so never visited by the collision pass, yet moving)."""
+    w = scenes.World(name="writeback")
+    scenes._floor(w)
+    cube = w.shape(scenes.cube_corners())
+    w.add_body(cube, (1, 1, 1), (0.0, -0.001, 0.0), (0, 0, 0, 1))
+    w.add_body(w.sphere(0.6), (1, 1, 1), (3.0, 2.0, 0.5), (0, 0, 0, 1), angular_velocity=(0.0, 0.0, -2.0))
+    left = w.shape(scenes.box_corners((0.4, 0.4, 0.4), (-0.6, 0.0, 0.0)))
+    right = w.shape(scenes.box_corners((0.4, 0.4, 0.4), (0.6, 0.0, 0.0)))
+    w.add_body((left, right), (1, 1, 1), (-3.0, 1.5, 0.0), scenes.quaternion_new((0.3, 1.0, 0.2), 25.0))
+    w.add_body(cube, (1, 1, 1), (0.0, 500.0, 0.0), (0, 0, 0, 1), angular_velocity=(0.4, 0.0, 0.3))
+    w.end_scene()
+    return w
+
+
+@pytest.mark.parametrize("make,frames", [(_writeback_scene, 100),
+                                         (lambda: scenes.batched_mixed(1, first_seed=11), 90)])
+def test_dropin_writes_back_collider_geometry(make, frames):
+    """After every call the entities' colliders hold what the reference's
+    collision pass left in them, bit for bit: transformed hull vertices,
+    transformed face normals and sphere centers at the pose after the last
+    substep's integration for the entities the pass visited, and the frame
+    driver's own update (the frame-start pose) for the others."""
+    a, b = _run_pair(make(), frames, worlds=True)
+    assert not state_bits_equal(a.state(), b.state())
+    for g, (x, y) in enumerate(zip(_geometry(a), _geometry(b))):
+        assert x.shape == y.shape
+        bad = np.flatnonzero(x.view(np.uint64) != y.view(np.uint64))
+        assert bad.size == 0, f"body {g}: {bad.size} values differ, first at {bad[0]}"
+    if make is _writeback_scene:
+        st = a.state()
+        assert st["active"][1, 0] == 0, "the resting cube should be asleep"
+        assert st["position"][4, 1] < 490.0, "the far cube should have fallen"
+
+
+def test_dropin_collider_geometry_untouched_without_collisions():
+    """With collisions off the reference never calls collider_update."""
+    a, b = _run_pair(scenes.chains(1, length=4, first_seed=5), 5, worlds=True, collisions=False)
+    for x, y in zip(_geometry(a), _geometry(b)):
+        assert np.array_equal(x.view(np.uint64), y.view(np.uint64))
+
+
+def test_model_matrices_match_entity_get_model_matrix():
+    """rp_world_model_matrices against the reference's entity_get_model_matrix
+    on the same states (scaled boxes, rotated bodies, spheres)."""
+    ref = load("ref")
+    ref.ref_world_model_matrix.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]
+    world = scenes.batched_uniform(2, first_seed=3)
+    sim = SimWorld("ref", world)
+    sim.step(30)
+    from _libs import pkg
+    gw = pkg.World(world)
+    gw.set_state(sim.state())
+    m = gw.model_matrices()
+    mf = gw.model_matrices(as_float=True)
+    for g in range(world.num_bodies):
+        r = np.zeros(16)
+        ref.ref_world_model_matrix(sim.h, g, r.ctypes.data)
+        assert np.array_equal(m[g].reshape(-1).view(np.uint64), r.view(np.uint64)), g
+        assert np.array_equal(mf[g].reshape(-1), r.astype(np.float32)), g
