@@ -164,6 +164,9 @@ RP_HD V3 transform_vertex(const M4& M, V3 v) {
     double y = v.x * M.m[1][0] + v.y * M.m[1][1] + v.z * M.m[1][2] + 1.0 * M.m[1][3];
     double z = v.x * M.m[2][0] + v.y * M.m[2][1] + v.z * M.m[2][2] + 1.0 * M.m[2][3];
     double ww = v.x * M.m[3][0] + v.y * M.m[3][1] + v.z * M.m[3][2] + 1.0 * M.m[3][3];
+    // A model matrix's last row is (0, 0, 0, 1), so ww is exactly 1 for a
+    // finite vertex; then 1/ww = 1 and each sc * t = t bit for bit.
+    if (ww == 1.0) return v3(x, y, z);
     double sc = 1.0 / ww;
     return v3(sc * x, sc * y, sc * z);
 }

@@ -107,6 +107,38 @@ constexpr int MAXC = 16;       // contacts kept per body pair
 // every G-th vertex, then reduce (larger dot wins, ties to the lower index),
 // which selects exactly the first maximum of the sequential scan; every lane
 // of the group returns the same index.
+
+// Lane exchanges inside a row of 16 lanes (DPP): partner = lane ^ 1, lane ^ 2
+// (quad permutations), 7 - lane within 8 (half mirror), 15 - lane (mirror).
+enum : int { DPP_XOR1 = 0xB1, DPP_XOR2 = 0x4E, DPP_HALF_MIRROR = 0x141, DPP_MIRROR = 0x140 };
+template <int CTRL> __device__ __forceinline__ int dpp_int(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL> __device__ __forceinline__ double dpp_double(double v) {
+    long long b = __double_as_longlong(v);
+    int lo = dpp_int<CTRL>((int)b), hi = dpp_int<CTRL>((int)(b >> 32));
+    return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
+}
+// (dot, index) of the partner lane, kept if its dot is larger or equal with a
+// lower index; an index < 0 means "none yet".
+template <int CTRL> __device__ __forceinline__ void argmax_step(double& m, int& sel) {
+    double om = dpp_double<CTRL>(m);
+    int os = dpp_int<CTRL>(sel);
+    if (os >= 0 && (sel < 0 || om > m || (om == m && os < sel))) {
+        m = om;
+        sel = os;
+    }
+}
+// First maximum over the G lanes of the group (G = 4, 8 or 16): after each
+// step every lane holds the best of a block of 2, 4, 8, 16 lanes.
+template <int G> __device__ __forceinline__ void group_argmax(double& m, int& sel) {
+    static_assert(G == 4 || G == 8 || G == 16, "lane groups of 4, 8 or 16");
+    argmax_step<DPP_XOR1>(m, sel);
+    argmax_step<DPP_XOR2>(m, sel);
+    if constexpr (G >= 8) argmax_step<DPP_HALF_MIRROR>(m, sel);
+    if constexpr (G >= 16) argmax_step<DPP_MIRROR>(m, sel);
+}
+
 template <int G = 1>
 RP_HD int support_index(const HullView& h, V3 d, uint32_t& err) {
     int sel = -1;
@@ -133,29 +165,22 @@ RP_HD int support_index(const HullView& h, V3 d, uint32_t& err) {
             double dd = dot(h.tv[i], d);
             if (dd > max_dot) { sel = i; max_dot = dd; }
         }
-#pragma unroll
-        for (int off = G / 2; off > 0; off >>= 1) {
-            double ob = __shfl_xor(max_dot, off, G);
-            int os = __shfl_xor(sel, off, G);
-            if (os >= 0 && (sel < 0 || ob > max_dot || (ob == max_dot && os < sel))) {
-                max_dot = ob;
-                sel = os;
-            }
-        }
+        group_argmax<G>(max_dot, sel);
     }
     if (sel < 0) { err |= ERR_SUPPORT_NAN; sel = 0; }
     return sel;
 }
-// Support indices of two hulls in directions d1 and d2 at once. For G = 1
-// the two scans advance together, eight vertices of each hull per batch, so
-// their memory latencies overlap; each scan keeps its own first-maximum
-// order. For G > 1 the two group scans run one after the other.
+// Support indices of two hulls in directions d1 and d2 at once: the two
+// scans advance together (eight vertices of each hull per batch for G = 1,
+// one vertex of each per lane and step for G > 1) so that their memory
+// latencies overlap; each scan keeps its own first-maximum order, and for
+// G > 1 the two group reductions run side by side.
 template <int G = 1>
 RP_HD void support_index2(const HullView& a, V3 d1, const HullView& b, V3 d2, int& i1, int& i2, uint32_t& err) {
+    int s1 = -1, s2 = -1;
+    double m1 = -DBL_MAX, m2 = -DBL_MAX;
+    int n = a.nv > b.nv ? a.nv : b.nv;
     if constexpr (G == 1) {
-        int s1 = -1, s2 = -1;
-        double m1 = -DBL_MAX, m2 = -DBL_MAX;
-        int n = a.nv > b.nv ? a.nv : b.nv;
         for (int i = 0; i < n; i += 8) {
             V3 va[8], vb[8];
 #pragma unroll
@@ -175,14 +200,25 @@ RP_HD void support_index2(const HullView& a, V3 d1, const HullView& b, V3 d2, in
                 }
             }
         }
-        if (s1 < 0) { err |= ERR_SUPPORT_NAN; s1 = 0; }
-        if (s2 < 0) { err |= ERR_SUPPORT_NAN; s2 = 0; }
-        i1 = s1;
-        i2 = s2;
     } else {
-        i1 = support_index<G>(a, d1, err);
-        i2 = support_index<G>(b, d2, err);
+        int gl = (int)(__lane_id() & (G - 1));
+        for (int i = gl; i < n; i += G) {
+            if (i < a.nv) {
+                double dd = dot(a.tv[i], d1);
+                if (dd > m1) { s1 = i; m1 = dd; }
+            }
+            if (i < b.nv) {
+                double dd = dot(b.tv[i], d2);
+                if (dd > m2) { s2 = i; m2 = dd; }
+            }
+        }
+        group_argmax<G>(m1, s1);
+        group_argmax<G>(m2, s2);
     }
+    if (s1 < 0) { err |= ERR_SUPPORT_NAN; s1 = 0; }
+    if (s2 < 0) { err |= ERR_SUPPORT_NAN; s2 = 0; }
+    i1 = s1;
+    i2 = s2;
 }
 
 template <int G = 1>
@@ -393,28 +429,37 @@ RP_HD bool epa_face(I3 f, const V3* poly, int np, V3& n_out, double& d_out) {
     return true;
 }
 
-// epa (epa.cpp:118-237): 0 = no convergence, 1 = converged, -1 = error
-template <int G = 1, class View = HullView>
-RP_HD int epa(const View& h1, const View& h2, const Simplex& s, V3& n_out, double& pen_out, uint32_t& err) {
-    V3 poly[EPA_MAXV];
-    I3 faces[EPA_MAXF];
-    V3 normals[EPA_MAXF];
-    double dist[EPA_MAXF];
-    int2 edges[EPA_MAXE];
-    int np = 4, nf = 4, ne = 0;
-    poly[0] = s.a; poly[1] = s.b; poly[2] = s.c; poly[3] = s.d;
-    faces[0] = {0, 1, 2}; faces[1] = {0, 2, 3}; faces[2] = {0, 3, 1}; faces[3] = {1, 2, 3};
-    V3 min_n = v3(0.0, 0.0, 0.0);
-    double min_d = DBL_MAX;
-    for (int i = 0; i < nf; ++i) {
-        V3 n; double d;
-        if (!epa_face(faces[i], poly, np, n, d)) { err |= ERR_EPA_DEGENERATE; return -1; }
-        normals[i] = n;
-        dist[i] = d;
-        if (d < min_d) { min_d = d; min_n = n; }
+// The EPA polytope (epa.cpp:118-237) over caller-owned arrays (separate
+// locals, so that the compiler can overlap their scratch with later phases).
+struct EpaPoly {
+    V3* poly;      // [EPA_MAXV]
+    I3* faces;     // [EPA_MAXF]
+    V3* normals;   // [EPA_MAXF]
+    double* dist;  // [EPA_MAXF]
+    int2* edges;   // [EPA_MAXE]
+    int np, nf, ne;
+    V3 min_n;
+    double min_d;
+
+    // The simplex's four faces and the closest one; false on a degenerate face.
+    RP_HD bool init(const Simplex& s, uint32_t& err) {
+        np = 4; nf = 4; ne = 0;
+        poly[0] = s.a; poly[1] = s.b; poly[2] = s.c; poly[3] = s.d;
+        faces[0] = {0, 1, 2}; faces[1] = {0, 2, 3}; faces[2] = {0, 3, 1}; faces[3] = {1, 2, 3};
+        min_n = v3(0.0, 0.0, 0.0);
+        min_d = DBL_MAX;
+        for (int i = 0; i < nf; ++i) {
+            V3 n; double d;
+            if (!epa_face(faces[i], poly, np, n, d)) { err |= ERR_EPA_DEGENERATE; return false; }
+            normals[i] = n;
+            dist[i] = d;
+            if (d < min_d) { min_d = d; min_n = n; }
+        }
+        return true;
     }
-    for (int it = 0; it < 100; ++it) {
-        V3 sp = support_md<G>(h1, h2, min_n, err);
+    // One iteration with the support point sp in direction min_n:
+    // 1 = converged (n_out, pen_out set), 2 = expanded, -1 = error.
+    RP_HD int step(V3 sp, V3& n_out, double& pen_out, uint32_t& err) {
         double d = dot(min_n, sp);
         if (fabs(d - min_d) < 0.0001) {
             n_out = min_n;
@@ -475,8 +520,67 @@ RP_HD int epa(const View& h1, const View& h2, const Simplex& s, V3& n_out, doubl
         for (int i = 0; i < nf; ++i)
             if (dist[i] < min_d) { min_d = dist[i]; min_n = normals[i]; }
         ne = 0;
+        return 2;
     }
-    return 0;
+};
+
+// Value of lane 0 of the caller's aligned group of G lanes.
+template <int G> __device__ __forceinline__ int group_bcast(int v) { return __shfl(v, 0, G); }
+template <int G> __device__ __forceinline__ double group_bcast(double v) { return __shfl(v, 0, G); }
+template <int G> __device__ __forceinline__ V3 group_bcast(V3 v) {
+    return v3(group_bcast<G>(v.x), group_bcast<G>(v.y), group_bcast<G>(v.z));
+}
+
+// epa (epa.cpp:118-237): 0 = no convergence, 1 = converged, -1 = error.
+// G > 1: the group's lanes compute each support point together; the
+// polytope lives in lane 0 alone (its private arrays), which expands it and
+// hands the next direction and the status to the group. Every lane returns
+// the status and, on convergence, lane 0's n_out / pen_out; the EPA error
+// bits are set in lane 0's err only.
+template <int G = 1, class View = HullView>
+RP_HD int epa(const View& h1, const View& h2, const Simplex& s, V3& n_out, double& pen_out, uint32_t& err) {
+    V3 poly[EPA_MAXV];
+    I3 faces[EPA_MAXF];
+    V3 normals[EPA_MAXF];
+    double dist[EPA_MAXF];
+    int2 edges[EPA_MAXE];
+    EpaPoly P;
+    P.poly = poly; P.faces = faces; P.normals = normals; P.dist = dist; P.edges = edges;
+    if constexpr (G == 1) {
+        if (!P.init(s, err)) return -1;
+        for (int it = 0; it < 100; ++it) {
+            V3 sp = support_md<G>(h1, h2, P.min_n, err);
+            int st = P.step(sp, n_out, pen_out, err);
+            if (st != 2) return st;
+        }
+        return 0;
+    } else {
+        bool leader = (__lane_id() & (G - 1)) == 0;
+        int st = 2;
+        V3 dir = v3(0.0, 0.0, 0.0);
+        if (leader) {
+            if (!P.init(s, err)) st = -1;
+            dir = P.min_n;
+        }
+        st = group_bcast<G>(st);
+        if (st != 2) return st;
+        dir = group_bcast<G>(dir);
+        for (int it = 0; it < 100; ++it) {
+            V3 sp = support_md<G>(h1, h2, dir, err);
+            if (leader) {
+                st = P.step(sp, n_out, pen_out, err);
+                dir = P.min_n;
+            }
+            st = group_bcast<G>(st);
+            if (st == 1) {
+                n_out = group_bcast<G>(n_out);
+                pen_out = group_bcast<G>(pen_out);
+            }
+            if (st != 2) return st;
+            dir = group_bcast<G>(dir);
+        }
+        return 0;
+    }
 }
 
 // ---------------------------------------------------------------------------
