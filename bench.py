@@ -216,9 +216,12 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tune-from", type=int, default=90, help="frame the autotuner's trials start from")
     ap.add_argument("--tune-frames", type=int, default=6)
-    ap.add_argument("--narrow-lanes", type=int, default=0, help="fixed narrow schedule (lanes per pair); skips the autotuner")
-    ap.add_argument("--manifold-lanes", type=int, default=0, help="fixed manifold schedule (lanes per pair); skips the autotuner")
-    ap.add_argument("--solve-lanes", type=int, default=0, help="fixed solve schedule (lanes per scene); skips the autotuner")
+    ap.add_argument("--narrow-lanes", type=int, default=None,
+                    help="fixed narrow schedule (lanes per pair); skips the autotuner")
+    ap.add_argument("--manifold-lanes", type=int, default=None,
+                    help="fixed manifold schedule (lanes per pair; 0 = built in the narrow kernel); skips the autotuner")
+    ap.add_argument("--solve-lanes", type=int, default=None,
+                    help="fixed solve schedule (lanes per scene); skips the autotuner")
     ap.add_argument("--eager", action="store_true", help="launch each kernel eagerly instead of replaying frame graphs")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurement")
     ap.add_argument("--adjoint-scenes", type=int, default=2048)
@@ -262,13 +265,14 @@ def main():
     tune = None
     if args.eager:
         gw.set_graphs(False)
-    if args.narrow_lanes:
+    fixed = [a is not None for a in (args.narrow_lanes, args.manifold_lanes, args.solve_lanes)]
+    if args.narrow_lanes is not None:
         gw.set_narrow_group(args.narrow_lanes)
-    if args.manifold_lanes:
+    if args.manifold_lanes is not None:
         gw.set_manifold_group(args.manifold_lanes)
-    if args.solve_lanes:
+    if args.solve_lanes is not None:
         gw.set_solve_lanes(args.solve_lanes)
-    if not (args.no_autotune or args.narrow_lanes or args.manifold_lanes or args.solve_lanes):
+    if not (args.no_autotune or any(fixed)):
         gw.step(args.tune_from)
         tune = gw.autotune(frames=args.tune_frames)
         gw.set_state(init)
@@ -344,20 +348,26 @@ def main():
         if oc is not None:
             scale = args.scenes / oc["scenes"]
             kernels = {}
+            fused = manifold_group == 0  # the narrow kernel builds the manifolds too
+            ops = dict(oc["ops"])
+            if fused:
+                ops["narrow"] = ops["narrow"] + ops["manifold"]
+                ops["manifold"] = 0.0
             for cls in ("integrate", "transform", "narrow", "manifold", "solve"):
                 ms, n = ktimes[cls]
-                flops = oc["ops"][cls] * scale
+                flops = ops[cls] * scale
                 kernels[cls] = {"ms_total": ms, "launches": n, "avg_us": 1e3 * ms / max(n, 1),
                                 "algorithmic_flops": flops,
                                 "tflops": flops / (ms * 1e-3) / 1e12 if ms > 0 else None}
             # the dominant kernel: largest share of the frame
-            names = {"narrow": "k_gjk_epa (GJK + EPA per live pair)",
+            names = {"narrow": ("k_gjk_epa (GJK + EPA + contact manifold per live pair)" if fused
+                                else "k_gjk_epa (GJK + EPA per live pair)"),
                      "manifold": "k_manifold (contact manifold per colliding pair)",
                      "solve": "k_solve (levelled position + velocity solve)"}
             dom = max(names, key=lambda c: kernels[c]["ms_total"])
             k = kernels[dom]
-            unit_ops = {"narrow": (oc["ops"]["narrow"], oc["narrow_pairs"], "live pair"),
-                        "manifold": (oc["ops"]["manifold"], oc["narrow_pairs"], "live pair"),
+            unit_ops = {"narrow": (ops["narrow"], oc["narrow_pairs"], "live pair"),
+                        "manifold": (ops["manifold"], oc["narrow_pairs"], "live pair"),
                         "solve": (oc["ops"]["solve"], oc["body_substeps"], "body-substep")}[dom]
             traffic = committed_traffic(dom, {"narrow": narrow_group, "manifold": manifold_group,
                                               "solve": solve_lanes}[dom])

@@ -306,10 +306,12 @@ uint64_t rp_world_adjoint_mismatches(rp_world* w);
 /* Narrow-phase schedules: lanes cooperating on one body pair in the GJK + EPA
  * kernel (narrow group) and in the contact-manifold kernel (manifold group):
  * 1 = one lane per pair over pre-transformed hulls; 4, 8 or 16 = a lane group
- * per pair with hull vertices staged in LDS and split support searches. All
- * schedules give identical results. Return 0, or -1 if the schedule does not
- * fit this world (LDS). rp_world_create picks a narrow group of 8 where it
- * fits and a manifold group of 1. */
+ * per pair with hull vertices staged in LDS and split support searches and
+ * face / edge selections. Manifold group 0: no manifold kernel; the GJK + EPA
+ * kernel builds each converged pair's manifold itself on the hulls it staged.
+ * All schedules give identical results. Return 0, or -1 if the schedule does
+ * not fit this world (LDS). rp_world_create picks a narrow group of 8 (else
+ * 16, 4) where it fits with manifold group 0, else 1 and 1. */
 int32_t rp_world_set_narrow_group(rp_world* w, int32_t lanes_per_pair);
 int32_t rp_world_narrow_group(const rp_world* w);
 int32_t rp_world_set_manifold_group(rp_world* w, int32_t lanes_per_pair);

@@ -205,26 +205,31 @@ class World:
             raise ValueError(f"narrow-phase schedule {lanes_per_pair} does not fit this world")
 
     def set_manifold_group(self, lanes_per_pair: int):
-        """Contact-manifold schedule: 1, 4, 8 or 16 lanes per body pair (same results)."""
+        """Contact-manifold schedule: 1, 4, 8 or 16 lanes per body pair in a
+        kernel of its own, or 0 = built by the GJK + EPA kernel on the hulls it
+        staged (same results)."""
         if self.lib.rp_world_set_manifold_group(self.h, lanes_per_pair) != 0:
             raise ValueError(f"manifold schedule {lanes_per_pair} does not fit this world")
 
-    def autotune(self, frames: int = 6, narrow=(1, 4, 8, 16), manifold=(1, 4, 8, 16), solve=(64, 32, 16),
-                 **step_kw) -> dict:
-        """Picks the GJK + EPA, manifold and solve schedules from per-kernel
-        times: each trial runs `frames` frames from the current state in
-        profiling mode with one candidate of each kind, and the state is
-        restored after each trial. A one-lane-per-pair schedule is charged the
-        vertex transform it needs. All schedules give identical results.
-        Returns {kind: {candidate: ms}}."""
+    NARROW_SCHEDULES = ((8, 0), (16, 0), (4, 0), (8, 1), (16, 1), (4, 1), (1, 1), (8, 8))
+
+    def autotune(self, frames: int = 6, schedules=NARROW_SCHEDULES, solve=(64, 32, 16), **step_kw) -> dict:
+        """Picks the narrow-phase schedule (GJK + EPA lanes per pair, manifold
+        lanes per pair or 0 for the fused kernel) and the solve schedule from
+        per-kernel times: each trial runs `frames` frames from the current
+        state in profiling mode, and the state is restored after each trial.
+        A schedule is charged its GJK + EPA, manifold and vertex-transform
+        kernels. All schedules give identical results.
+        Returns {"narrow": {"g/m": ms}, "solve": {lanes: ms}}."""
         start = self.state()
-        narrow = [g for g in narrow if self.lib.rp_world_set_narrow_group(self.h, g) == 0]
-        manifold = [g for g in manifold if self.lib.rp_world_set_manifold_group(self.h, g) == 0]
+        ok = []
+        for g, m in schedules:
+            if self.lib.rp_world_set_narrow_group(self.h, g) == 0 and self.lib.rp_world_set_manifold_group(self.h, m) == 0:
+                ok.append((g, m))
         solve = [n for n in solve if self.lib.rp_world_set_solve_lanes(self.h, n) == 0]
-        out = {"narrow": {}, "manifold": {}, "solve": {}}
-        trials = max(len(narrow), len(manifold), len(solve))
-        for t in range(trials):
-            g, m, n = narrow[t % len(narrow)], manifold[t % len(manifold)], solve[t % len(solve)]
+        out = {"narrow": {}, "solve": {}}
+        for t in range(max(len(ok), len(solve))):
+            (g, m), n = ok[t % len(ok)], solve[t % len(solve)]
             self.set_narrow_group(g)
             self.set_manifold_group(m)
             self.set_solve_lanes(n)
@@ -235,12 +240,11 @@ class World:
             kt = self.kernel_times()
             self.set_profiling(False)
             self.set_state(start)
-            xf = kt["transform"][0]
-            out["narrow"].setdefault(g, kt["narrow"][0] + (xf if g == 1 else 0.0))
-            out["manifold"].setdefault(m, kt["manifold"][0] + (xf if m == 1 else 0.0))
+            out["narrow"].setdefault(f"{g}/{m}", kt["narrow"][0] + kt["manifold"][0] + kt["transform"][0])
             out["solve"].setdefault(n, kt["solve"][0])
-        self.set_narrow_group(min(out["narrow"], key=out["narrow"].get))
-        self.set_manifold_group(min(out["manifold"], key=out["manifold"].get))
+        g, m = map(int, min(out["narrow"], key=out["narrow"].get).split("/"))
+        self.set_narrow_group(g)
+        self.set_manifold_group(m)
         self.set_solve_lanes(min(out["solve"], key=out["solve"].get))
         return out
 

@@ -10,6 +10,7 @@ Flags that matter for results:
 from __future__ import annotations
 
 import os
+import re
 import subprocess
 import sys
 from pathlib import Path
@@ -30,14 +31,50 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
+# The runtime sizes a kernel's scratch (private) memory for every wave the
+# device can hold at the kernel's occupancy: bytes per lane x 64 lanes x
+# waves per SIMD x 4 SIMDs x 256 CUs. A dispatch whose total exceeds 4 GiB
+# fails with HSA_STATUS_ERROR_OUT_OF_RESOURCES, so the build refuses one.
+SCRATCH_LIMIT = 4 << 30
+DEVICE_SIMDS = 4 * 256
+
+
+def kernel_resources(remarks: str) -> dict:
+    """{kernel: (scratch bytes per lane, waves per SIMD)} from hipcc's
+    kernel-resource-usage remarks."""
+    out, name, scratch = {}, None, None
+    for line in remarks.splitlines():
+        m = re.search(r"Function Name: (\S+)", line)
+        if m:
+            name, scratch = m.group(1), None
+            continue
+        m = re.search(r"ScratchSize \[bytes/lane\]: (\d+)", line)
+        if m and name:
+            scratch = int(m.group(1))
+            continue
+        m = re.search(r"Occupancy \[waves/SIMD\]: (\d+)", line)
+        if m and name and scratch is not None:
+            out[name] = (scratch, int(m.group(1)))
+            name = None
+    return out
+
+
 def build(verbose: bool = False) -> Path:
-    cmd = [HIPCC, f"--offload-arch={ARCH}", *FLAGS, "-o", str(LIB), *map(str, SOURCES)]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", *FLAGS, "-Rpass-analysis=kernel-resource-usage",
+           "-o", str(LIB), *map(str, SOURCES)]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError(f"hipcc failed with exit code {r.returncode}")
+    res = kernel_resources(r.stderr)
+    if not res:
+        raise RuntimeError("hipcc printed no kernel resource usage")
+    over = {k: v for k, v in res.items() if v[0] * 64 * v[1] * DEVICE_SIMDS > SCRATCH_LIMIT}
+    if over:
+        LIB.unlink(missing_ok=True)
+        raise RuntimeError(f"kernels whose scratch exceeds the runtime's limit at their occupancy: {over}")
     return LIB
 
 

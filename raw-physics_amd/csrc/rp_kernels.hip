@@ -313,15 +313,34 @@ __device__ __forceinline__ void pair_views(const DevWorld& w, int g, V3* st, int
     if constexpr (G > 1) {
         const V3* lv1 = w.hull_v + w.hull_v_off[w.body_hull[b1]];
         const V3* lv2 = w.hull_v + w.hull_v_off[w.body_hull[b2]];
-        for (int i = gl; i < h1.nv; i += G) st[i] = transform_vertex(w.model[b1], lv1[i]);
-        for (int i = gl; i < h2.nv; i += G) st[h1.nv + i] = transform_vertex(w.model[b2], lv2[i]);
+        const M4 M1 = w.model[b1], M2 = w.model[b2];
+        // both hulls together, two vertices of each per lane and batch, all
+        // loaded before any is transformed
+        int n = h1.nv > h2.nv ? h1.nv : h2.nv;
+        for (int i = gl; i < n; i += 2 * G) {
+            V3 va[2], vb[2];
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                int j = i + k * G;
+                if (j < h1.nv) va[k] = lv1[j];
+                if (j < h2.nv) vb[k] = lv2[j];
+            }
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                int j = i + k * G;
+                if (j < h1.nv) st[j] = transform_vertex(M1, va[k]);
+                if (j < h2.nv) st[h1.nv + j] = transform_vertex(M2, vb[k]);
+            }
+        }
         group_lds_sync();
         h1.tv = st;
         h2.tv = st + h1.nv;
     }
 }
 
-template <int G>
+// kManifold: the contact manifold of a converged pair is built here too, on
+// the same staged hulls, and its contacts written (no manifold queue).
+template <int G, bool kManifold>
 __global__ void __launch_bounds__(64) k_gjk_epa(DevWorld w) {
     extern __shared__ V3 rp_stage[];
     constexpr int NG = 64 / G;
@@ -342,12 +361,26 @@ __global__ void __launch_bounds__(64) k_gjk_epa(DevWorld w) {
         int r = 0;
         bool hit = gjk<G>(h1, h2, sx, err, cycled);
         if (hit) r = epa<G>(h1, h2, sx, n, pen, err);
+        if constexpr (kManifold) {
+            ContactPt cts[MAXC];
+            int nc = 0;
+            if (r == 1) nc = manifold<G>(h1, h2, n, cts, err);
+            if (gl == 0) {
+                const Body& e1 = w.body[b1];
+                const Body& e2 = w.body[b2];
+                Contact* out = w.contacts + (size_t)g * MAXC;
+                for (int c = 0; c < nc; ++c) contact_to_constraint(e1, e2, cts[c], out[c]);
+                w.ncontact[g] = nc;
+            }
+        }
         if (gl == 0) {
-            if (r == 1) {
-                w.pair_normal[g] = n;
-                w.mlist[atomicAdd(&w.counters[CNT_MANIFOLD], 1)] = g;
-            } else {
-                w.ncontact[g] = 0;
+            if constexpr (!kManifold) {
+                if (r == 1) {
+                    w.pair_normal[g] = n;
+                    w.mlist[atomicAdd(&w.counters[CNT_MANIFOLD], 1)] = g;
+                } else {
+                    w.ncontact[g] = 0;
+                }
             }
             if (err) atomicOr(&w.scene_err[s], err);
             if (hit && r != 1) atomicAdd(&w.stats[STAT_EPA_FAIL], 1ull);  // epa.cpp:233-236
@@ -653,6 +686,7 @@ hipError_t launch_frame(const DevWorld& w, const FrameParams& fp, hipStream_t st
         if (fp.keep_pose && sub + 1 == fp.substeps)
             hipLaunchKernelGGL(k_keep_pose, dim3(cdiv(w.nb, 256)), dim3(256), 0, st, w);
         if (fp.collisions) {
+            bool fused = w.manifold_group == 0;
             if (w.narrow_group == 1 || w.manifold_group == 1 || w.has_general) {
                 t.begin(KC_TRANSFORM, st);
                 hipLaunchKernelGGL(k_transform, dim3(cdiv(w.n_xf, 256)), dim3(256), 0, st, w, 0);
@@ -661,24 +695,35 @@ hipError_t launch_frame(const DevWorld& w, const FrameParams& fp, hipStream_t st
             t.begin(KC_NARROW, st);
             size_t lds = w.narrow_group > 1 ? (size_t)(64 / w.narrow_group) * w.stage_cap * sizeof(V3) : 0;
             dim3 nb(w.narrow_blocks), t64(64);
-            switch (w.narrow_group) {
-            case 4: hipLaunchKernelGGL(k_gjk_epa<4>, nb, t64, lds, st, w); break;
-            case 8: hipLaunchKernelGGL(k_gjk_epa<8>, nb, t64, lds, st, w); break;
-            case 16: hipLaunchKernelGGL(k_gjk_epa<16>, nb, t64, lds, st, w); break;
-            default: hipLaunchKernelGGL(k_gjk_epa<1>, nb, t64, 0, st, w); break;
+            if (fused) {
+                switch (w.narrow_group) {
+                case 4: hipLaunchKernelGGL((k_gjk_epa<4, true>), nb, t64, lds, st, w); break;
+                case 8: hipLaunchKernelGGL((k_gjk_epa<8, true>), nb, t64, lds, st, w); break;
+                case 16: hipLaunchKernelGGL((k_gjk_epa<16, true>), nb, t64, lds, st, w); break;
+                default: hipLaunchKernelGGL((k_gjk_epa<1, true>), nb, t64, 0, st, w); break;
+                }
+            } else {
+                switch (w.narrow_group) {
+                case 4: hipLaunchKernelGGL((k_gjk_epa<4, false>), nb, t64, lds, st, w); break;
+                case 8: hipLaunchKernelGGL((k_gjk_epa<8, false>), nb, t64, lds, st, w); break;
+                case 16: hipLaunchKernelGGL((k_gjk_epa<16, false>), nb, t64, lds, st, w); break;
+                default: hipLaunchKernelGGL((k_gjk_epa<1, false>), nb, t64, 0, st, w); break;
+                }
             }
             if (w.has_general) hipLaunchKernelGGL(k_narrow_general, dim3(1024), t64, 0, st, w);
             t.end(st);
-            t.begin(KC_MANIFOLD, st);
-            lds = w.manifold_group > 1 ? (size_t)(64 / w.manifold_group) * w.stage_cap * sizeof(V3) : 0;
-            dim3 mb(w.manifold_blocks);
-            switch (w.manifold_group) {
-            case 4: hipLaunchKernelGGL(k_manifold<4>, mb, t64, lds, st, w); break;
-            case 8: hipLaunchKernelGGL(k_manifold<8>, mb, t64, lds, st, w); break;
-            case 16: hipLaunchKernelGGL(k_manifold<16>, mb, t64, lds, st, w); break;
-            default: hipLaunchKernelGGL(k_manifold<1>, mb, t64, 0, st, w); break;
+            if (!fused) {
+                t.begin(KC_MANIFOLD, st);
+                lds = w.manifold_group > 1 ? (size_t)(64 / w.manifold_group) * w.stage_cap * sizeof(V3) : 0;
+                dim3 mb(w.manifold_blocks);
+                switch (w.manifold_group) {
+                case 4: hipLaunchKernelGGL(k_manifold<4>, mb, t64, lds, st, w); break;
+                case 8: hipLaunchKernelGGL(k_manifold<8>, mb, t64, lds, st, w); break;
+                case 16: hipLaunchKernelGGL(k_manifold<16>, mb, t64, lds, st, w); break;
+                default: hipLaunchKernelGGL(k_manifold<1>, mb, t64, 0, st, w); break;
+                }
+                t.end(st);
             }
-            t.end(st);
             if (rec) hipLaunchKernelGGL(k_rec_contacts, dim3(256), dim3(256), 0, st, w, *rec, (int)sub);
         }
         t.begin(KC_SOLVE, st);

@@ -201,15 +201,28 @@ RP_HD void support_index2(const HullView& a, V3 d1, const HullView& b, V3 d2, in
             }
         }
     } else {
+        // a lane visits its vertices gl, gl + G, ... in order, four of each
+        // hull per batch, all loaded before any is compared
         int gl = (int)(__lane_id() & (G - 1));
-        for (int i = gl; i < n; i += G) {
-            if (i < a.nv) {
-                double dd = dot(a.tv[i], d1);
-                if (dd > m1) { s1 = i; m1 = dd; }
+        for (int i = gl; i < n; i += 4 * G) {
+            V3 va[4], vb[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                int j = i + k * G;
+                if (j < a.nv) va[k] = a.tv[j];
+                if (j < b.nv) vb[k] = b.tv[j];
             }
-            if (i < b.nv) {
-                double dd = dot(b.tv[i], d2);
-                if (dd > m2) { s2 = i; m2 = dd; }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                int j = i + k * G;
+                if (j < a.nv) {
+                    double dd = dot(va[k], d1);
+                    if (dd > m1) { s1 = j; m1 = dd; }
+                }
+                if (j < b.nv) {
+                    double dd = dot(vb[k], d2);
+                    if (dd > m2) { s2 = j; m2 = dd; }
+                }
             }
         }
         group_argmax<G>(m1, s1);
@@ -665,19 +678,79 @@ RP_HD bool skew_lines(V3 p1, V3 d1, V3 p2, V3 d2, V3& l1, V3& l2) {
 struct ContactPt { V3 p1, p2, n; };
 
 // convex_convex_contact_manifold (clipping.cpp:249-341). Returns the number
-// of contacts written to `out` (at most MAXC).
+// of contacts written to `out` (at most MAXC). G > 1: the group's lanes find
+// the support points, the best faces and the best edge pair together (split
+// scans, first-maximum reductions); clipping and the contact points run on
+// lane 0 of the group alone, whose `out`, return value and error bits are
+// the results (the other lanes return 0).
+// get_face_with_most_fitting_normal over the group: lane k scores the
+// entries k, k + G, ... of the vertex's face list, then the group keeps the
+// first maximum (larger projection, ties to the earlier entry).
+template <int G>
+__device__ __forceinline__ int best_face_group(int sidx, const HullView& h, V3 n, uint32_t& err) {
+    int gl = (int)(__lane_id() & (G - 1));
+    int b = h.vf_off[sidx], cnt = h.vf_off[sidx + 1] - b;
+    double max_proj = -DBL_MAX;
+    int pos = -1;
+    for (int k = gl; k < cnt; k += G) {
+        double proj = dot(face_normal_world(h, h.vf[b + k]), n);
+        if (proj > max_proj) { max_proj = proj; pos = k; }
+    }
+    group_argmax<G>(max_proj, pos);
+    if (pos < 0) { err |= ERR_SELECT; return 0; }
+    return h.vf[b + pos];
+}
+
 template <int G = 1>
 RP_HD int manifold(const HullView& h1, const HullView& h2, V3 normal, ContactPt* out, uint32_t& err) {
     const double EPSILON = 0.0001;
     V3 inormal = inv3(normal);
     int s1, s2;
     support_index2<G>(h1, normal, h2, inormal, s1, s2, err);
-    int f1 = best_face(s1, h1, normal, err);
-    int f2 = best_face(s2, h2, inormal, err);
+    int f1, f2;
+    if constexpr (G > 1) {
+        f1 = best_face_group<G>(s1, h1, normal, err);
+        f2 = best_face_group<G>(s2, h2, inormal, err);
+    } else {
+        f1 = best_face(s1, h1, normal, err);
+        f2 = best_face(s2, h2, inormal, err);
+    }
     // get_edge_with_most_fitting_normal (clipping.cpp:154-200)
     V3 edge_n = v3(0.0, 0.0, 0.0);
     int e1a = 0, e1b = 0, e2a = 0, e2b = 0;
-    {
+    if constexpr (G > 1) {
+        // candidates in the sequential order (neighbour i of s1, neighbour j
+        // of s2, then n before -n) are numbered 2 (i n2 + j) + variant; lane k
+        // scores the pairs k, k + G, ... and the group keeps the first maximum
+        int gl = (int)(__lane_id() & (G - 1));
+        V3 sp1 = h1.tv[s1], sp2 = h2.tv[s2];
+        int i0 = h1.vn_off[s1], n1 = h1.vn_off[s1 + 1] - i0;
+        int j0 = h2.vn_off[s2], n2 = h2.vn_off[s2 + 1] - j0;
+        double max_dot = -DBL_MAX;
+        int best = -1;
+        for (int k = gl; k < n1 * n2; k += G) {
+            int a = k / n2, b = k - a * n2;
+            V3 ed1 = sub(sp1, h1.tv[h1.vn[i0 + a]]);
+            V3 ed2 = sub(sp2, h2.tv[h2.vn[j0 + b]]);
+            V3 cn = nrm(cross(ed1, ed2));
+            V3 cni = inv3(cn);
+            double dd = dot(cn, normal);
+            if (dd > max_dot) { max_dot = dd; best = 2 * k; }
+            dd = dot(cni, normal);
+            if (dd > max_dot) { max_dot = dd; best = 2 * k + 1; }
+        }
+        group_argmax<G>(max_dot, best);
+        if (best < 0) {
+            err |= ERR_SELECT;
+        } else {
+            int k = best >> 1, a = k / n2, b = k - a * n2;
+            e1a = s1; e1b = h1.vn[i0 + a]; e2a = s2; e2b = h2.vn[j0 + b];
+            V3 cn = nrm(cross(sub(sp1, h1.tv[e1b]), sub(sp2, h2.tv[e2b])));
+            edge_n = (best & 1) ? inv3(cn) : cn;
+        }
+        // the rest (clipping, contact points) runs on lane 0 of the group
+        if (gl != 0) return 0;
+    } else {
         V3 sp1 = h1.tv[s1], sp2 = h2.tv[s2];
         double max_dot = -DBL_MAX;
         bool found = false;

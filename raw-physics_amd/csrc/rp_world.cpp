@@ -482,8 +482,10 @@ static bool build_image(const HostWorld& hw, DeviceImage& im) {
     for (int g : {8, 16, 4})
         if (narrow_group_supported(d, g)) { d.narrow_group = g; break; }
     d.narrow_blocks = narrow_blocks_for(d, d.narrow_group, pcap);
-    d.manifold_group = 1;
-    d.manifold_blocks = narrow_blocks_for(d, d.manifold_group, pcap);
+    // manifolds built by the GJK + EPA kernel on its staged hulls where it
+    // stages them, else by the one-lane-per-pair manifold kernel
+    d.manifold_group = d.narrow_group > 1 ? 0 : 1;
+    d.manifold_blocks = narrow_blocks_for(d, 1, pcap);
     d.solve_lanes = 64;
 
     bool ok = true;
@@ -1193,10 +1195,10 @@ int32_t rp_world_set_narrow_group(rp_world* w, int32_t lanes_per_pair) {
 int32_t rp_world_narrow_group(const rp_world* w) { return w->im.d.narrow_group; }
 
 int32_t rp_world_set_manifold_group(rp_world* w, int32_t lanes_per_pair) {
-    if (!narrow_group_supported(w->im.d, lanes_per_pair)) return -1;
+    if (lanes_per_pair != 0 && !narrow_group_supported(w->im.d, lanes_per_pair)) return -1;
     (void)hipStreamSynchronize(w->stream);
     w->im.d.manifold_group = lanes_per_pair;
-    w->im.d.manifold_blocks = narrow_blocks_for(w->im.d, lanes_per_pair, w->im.pair_capacity);
+    w->im.d.manifold_blocks = narrow_blocks_for(w->im.d, lanes_per_pair ? lanes_per_pair : 1, w->im.pair_capacity);
     ++w->sched_gen;
     return 0;
 }

@@ -99,17 +99,20 @@ def test_batched_ico162_bit_exact():
     _pair(scenes.batched_uniform(3, ico_subdiv=2), frames=60, check_every=20)
 
 
-@pytest.mark.parametrize("kind", ["narrow", "manifold"])
+@pytest.mark.parametrize("kind", ["narrow", "manifold", "fused"])
 @pytest.mark.parametrize("lanes", [1, 4, 8, 16])
 def test_every_narrow_schedule_bit_exact(kind, lanes):
     """The GJK + EPA kernel's and the manifold kernel's schedules (one lane per
-    pair, or 4/8/16 lanes per pair with split support searches) give
-    identical results."""
-    setter = "rp_world_set_narrow_group" if kind == "narrow" else "rp_world_set_manifold_group"
+    pair, or 4/8/16 lanes per pair with split support searches), and the
+    GJK + EPA kernel building the manifolds itself (manifold schedule 0),
+    give identical results."""
+    setter = "rp_world_set_manifold_group" if kind == "manifold" else "rp_world_set_narrow_group"
     for world, frames in ((scenes.batched_uniform(6, first_seed=500), 90),
                           (scenes.batched_uniform(2, first_seed=600, ico_subdiv=2), 40)):
         o = SimWorld("orc", world)
         g = SimWorld("rp", world)
+        if kind == "fused":
+            assert g.lib.rp_world_set_manifold_group(g.h, 0) == 0
         # a group stages both hulls of its pair in LDS; schedules whose
         # (64 / lanes) groups would need more than 64 KiB are refused
         max_nv = max(len(np.unique(c, axis=0)) for c in world.shapes)
