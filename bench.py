@@ -115,15 +115,16 @@ def op_counts(scenes_mod, n_scenes: int, frames: int, start: int = 0):
 TRAFFIC_FILE = ROOT / "profiles" / "traffic_window_f300_batched_uniform_4096x16.json"
 
 
-def committed_traffic(kernel_class: str, lanes: int):
+def committed_traffic(kernel_class: str, schedule: dict):
     """HBM bytes per launch of a kernel from the committed rocprofv3 PMC
     summary (FETCH_SIZE and WRITE_SIZE passes, scaled by the copy-kernel
-    calibration), when it covers this kernel and schedule; else None."""
+    calibration), when it covers this kernel under the run's schedules
+    ({"narrow": lanes, "manifold": lanes or 0, "solve": lanes}); else None."""
     if not TRAFFIC_FILE.exists():
         return None
     d = json.loads(TRAFFIC_FILE.read_text())
     k = d.get("kernels", {}).get(kernel_class)
-    if not k or k.get("lanes") != lanes:
+    if not k or d.get("schedule") != schedule:
         return None
     return {"bytes_per_launch": k["bytes_per_launch"], "source": f"{TRAFFIC_FILE.relative_to(ROOT)}: {d['window']}"}
 
@@ -370,7 +371,7 @@ def main():
                         "manifold": (ops["manifold"], oc["narrow_pairs"], "live pair"),
                         "solve": (oc["ops"]["solve"], oc["body_substeps"], "body-substep")}[dom]
             traffic = committed_traffic(dom, {"narrow": narrow_group, "manifold": manifold_group,
-                                              "solve": solve_lanes}[dom])
+                                              "solve": solve_lanes})
             roofline = {"bound": "fp64-valu", "kernel": names[dom],
                         "achieved": k["tflops"], "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                         "frac": k["tflops"] / FP64_PEAK_TFLOPS if k["tflops"] else None,

@@ -31,11 +31,13 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
-# The runtime sizes a kernel's scratch (private) memory for every wave the
+# The runtime reserves a kernel's scratch (private) memory for every wave the
 # device can hold at the kernel's occupancy: bytes per lane x 64 lanes x
-# waves per SIMD x 4 SIMDs x 256 CUs. A dispatch whose total exceeds 4 GiB
-# fails with HSA_STATUS_ERROR_OUT_OF_RESOURCES, so the build refuses one.
-SCRATCH_LIMIT = 4 << 30
+# waves per SIMD x 4 SIMDs x 256 CUs, and keeps it per hardware queue.
+# Reservations of 2 to 4 GiB have failed with HSA_STATUS_ERROR_OUT_OF_RESOURCES
+# in a process that had already run other large ones, so the build refuses a
+# kernel above 2 GiB.
+SCRATCH_LIMIT = 2 << 30
 DEVICE_SIMDS = 4 * 256
 
 

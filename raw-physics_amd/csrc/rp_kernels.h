@@ -58,6 +58,8 @@ struct DevWorld {
     V3* pair_normal;                // [pair capacity] EPA normal of the pairs queued for the manifold
     int32_t* mlist;                 // [pair capacity] manifold queue: live-pair slots
     int32_t* counters;              // [CNT_COUNT] queue lengths of the current substep
+    char* group_work;               // [group_work_slots][GROUP_WORK_BYTES] lane-group working memory
+    int64_t group_work_slots;       // >= blocks x groups per block of the grouped narrow / manifold kernels
     Contact* contacts;              // [pair capacity * MAXC]
     // external constraints
     const ExtC* ext;                // [n_ext]
@@ -174,6 +176,8 @@ hipError_t launch_broad(const DevWorld& w, hipStream_t st, const BigScene* bigs 
 // entity_get_model_matrix of every body into out ([nb][4][4] row-major,
 // double or float), device memory.
 hipError_t launch_model_matrices(const DevWorld& w, void* out, bool f32, hipStream_t st);
+// Working-memory slots the grouped narrow-phase kernels of this schedule use.
+int64_t group_work_slots_needed(const DevWorld& w);
 // Lanes per pair the narrow phase can use for this world (LDS permitting).
 bool narrow_group_supported(const DevWorld& w, int g);
 int narrow_blocks_for(const DevWorld& w, int g, int64_t pair_capacity);

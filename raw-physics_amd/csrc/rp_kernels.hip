@@ -341,7 +341,7 @@ __device__ __forceinline__ void pair_views(const DevWorld& w, int g, V3* st, int
 // kManifold: the contact manifold of a converged pair is built here too, on
 // the same staged hulls, and its contacts written (no manifold queue).
 template <int G, bool kManifold>
-__global__ void __launch_bounds__(64) k_gjk_epa(DevWorld w) {
+__device__ __forceinline__ void gjk_epa_pairs(DevWorld& w) {
     extern __shared__ V3 rp_stage[];
     constexpr int NG = 64 / G;
     int grp = threadIdx.x / G, gl = threadIdx.x & (G - 1);
@@ -359,12 +359,14 @@ __global__ void __launch_bounds__(64) k_gjk_epa(DevWorld w) {
         V3 n = v3(0.0, 0.0, 0.0);
         double pen;
         int r = 0;
+        char* work = G > 1 ? w.group_work + (size_t)(blockIdx.x * NG + grp) * GROUP_WORK_BYTES : nullptr;
         bool hit = gjk<G>(h1, h2, sx, err, cycled);
-        if (hit) r = epa<G>(h1, h2, sx, n, pen, err);
+        if (hit) r = epa<G>(h1, h2, sx, n, pen, err, work);
         if constexpr (kManifold) {
-            ContactPt cts[MAXC];
+            ContactPt local[G == 1 ? MAXC : 1];
+            ContactPt* cts = G == 1 ? local : (ContactPt*)(work + CLIP_BYTES);
             int nc = 0;
-            if (r == 1) nc = manifold<G>(h1, h2, n, cts, err);
+            if (r == 1) nc = manifold<G>(h1, h2, n, cts, err, work);
             if (gl == 0) {
                 const Body& e1 = w.body[b1];
                 const Body& e2 = w.body[b2];
@@ -390,6 +392,17 @@ __global__ void __launch_bounds__(64) k_gjk_epa(DevWorld w) {
     }
 }
 
+// Lane groups (G = 4, 8, 16) at two waves per SIMD; one lane per pair (G = 1).
+template <int G, bool kManifold>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_gjk_epa(DevWorld w) {
+    static_assert(G > 1, "lane groups");
+    gjk_epa_pairs<G, kManifold>(w);
+}
+template <bool kManifold>
+__global__ void __launch_bounds__(64) k_gjk_epa_lane(DevWorld w) {
+    gjk_epa_pairs<1, kManifold>(w);
+}
+
 template <int G>
 __global__ void __launch_bounds__(64) k_manifold(DevWorld w) {
     extern __shared__ V3 rp_stage[];
@@ -403,8 +416,10 @@ __global__ void __launch_bounds__(64) k_manifold(DevWorld w) {
         HullView h1, h2;
         pair_views<G>(w, g, st, gl, s, b1, b2, h1, h2);
         uint32_t err = 0;
-        ContactPt cts[MAXC];
-        int nc = manifold<G>(h1, h2, w.pair_normal[g], cts, err);
+        char* work = G > 1 ? w.group_work + (size_t)(blockIdx.x * NG + grp) * GROUP_WORK_BYTES : nullptr;
+        ContactPt local[G == 1 ? MAXC : 1];
+        ContactPt* cts = G == 1 ? local : (ContactPt*)(work + CLIP_BYTES);
+        int nc = manifold<G>(h1, h2, w.pair_normal[g], cts, err, work);
         if (gl == 0) {
             const Body& e1 = w.body[b1];
             const Body& e2 = w.body[b2];
@@ -603,13 +618,18 @@ static void launch_solve(const DevWorld& w, const FrameParams& fp, hipStream_t s
 }
 
 // Contacts of one body pair at the current transformed poses (debug / parity).
-__global__ void k_pair_contacts(DevWorld w, int b1, int b2, double* out, int* count, uint32_t* err_out) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+// One lane group of 8 over the pre-transformed hulls, its polytope and
+// polygons in lane-group working memory slot 0 (no large private arrays).
+__global__ void __launch_bounds__(64) k_pair_contacts(DevWorld w, int b1, int b2, double* out, int* count,
+                                                     uint32_t* err_out) {
+    if (blockIdx.x != 0 || threadIdx.x >= 8) return;
     HullView h1 = hull_view(w, b1), h2 = hull_view(w, b2);
     uint32_t err = 0;
     int flags = 0;
-    ContactPt cts[MAXC];
-    int nc = pair_contacts(h1, h2, cts, err, flags);
+    char* work = w.group_work;
+    ContactPt* cts = (ContactPt*)(work + CLIP_BYTES);
+    int nc = pair_contacts<8>(h1, h2, cts, err, flags, work);
+    if (threadIdx.x != 0) return;
     for (int k = 0; k < nc; ++k) {
         double* o = out + 9 * k;
         o[0] = cts[k].p1.x; o[1] = cts[k].p1.y; o[2] = cts[k].p1.z;
@@ -700,14 +720,14 @@ hipError_t launch_frame(const DevWorld& w, const FrameParams& fp, hipStream_t st
                 case 4: hipLaunchKernelGGL((k_gjk_epa<4, true>), nb, t64, lds, st, w); break;
                 case 8: hipLaunchKernelGGL((k_gjk_epa<8, true>), nb, t64, lds, st, w); break;
                 case 16: hipLaunchKernelGGL((k_gjk_epa<16, true>), nb, t64, lds, st, w); break;
-                default: hipLaunchKernelGGL((k_gjk_epa<1, true>), nb, t64, 0, st, w); break;
+                default: hipLaunchKernelGGL(k_gjk_epa_lane<true>, nb, t64, 0, st, w); break;
                 }
             } else {
                 switch (w.narrow_group) {
                 case 4: hipLaunchKernelGGL((k_gjk_epa<4, false>), nb, t64, lds, st, w); break;
                 case 8: hipLaunchKernelGGL((k_gjk_epa<8, false>), nb, t64, lds, st, w); break;
                 case 16: hipLaunchKernelGGL((k_gjk_epa<16, false>), nb, t64, lds, st, w); break;
-                default: hipLaunchKernelGGL((k_gjk_epa<1, false>), nb, t64, 0, st, w); break;
+                default: hipLaunchKernelGGL(k_gjk_epa_lane<false>, nb, t64, 0, st, w); break;
                 }
             }
             if (w.has_general) hipLaunchKernelGGL(k_narrow_general, dim3(1024), t64, 0, st, w);
@@ -739,7 +759,7 @@ hipError_t launch_pair_contacts(const DevWorld& w, int b1, int b2, double* out, 
                                 hipStream_t st) {
     hipLaunchKernelGGL(k_model, dim3(cdiv(w.nb, 128)), dim3(128), 0, st, w);
     hipLaunchKernelGGL(k_transform, dim3(cdiv(w.n_xf, 256)), dim3(256), 0, st, w, 1);
-    hipLaunchKernelGGL(k_pair_contacts, dim3(1), dim3(1), 0, st, w, b1, b2, out, count, err);
+    hipLaunchKernelGGL(k_pair_contacts, dim3(1), dim3(64), 0, st, w, b1, b2, out, count, err);
     return hipGetLastError();
 }
 
@@ -754,6 +774,13 @@ hipError_t launch_fp64_selftest(const double* a, const double* b, double* out, i
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_fp64_selftest, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, b, out, n);
     return hipGetLastError();
+}
+
+int64_t group_work_slots_needed(const DevWorld& w) {
+    int64_t n = 1;  // k_pair_contacts
+    if (w.narrow_group > 1) n = std::max<int64_t>(n, (int64_t)w.narrow_blocks * (64 / w.narrow_group));
+    if (w.manifold_group > 1) n = std::max<int64_t>(n, (int64_t)w.manifold_blocks * (64 / w.manifold_group));
+    return n;
 }
 
 bool narrow_group_supported(const DevWorld& w, int g) {

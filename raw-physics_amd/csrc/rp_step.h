@@ -544,22 +544,45 @@ template <int G> __device__ __forceinline__ V3 group_bcast(V3 v) {
     return v3(group_bcast<G>(v.x), group_bcast<G>(v.y), group_bcast<G>(v.z));
 }
 
+// Working memory of the first lane of a lane group (G > 1): the EPA polytope,
+// then (reusing it) the clipping polygons and the contact points. A slice of
+// a device buffer, one per group, so that one lane's arrays are contiguous
+// (private arrays are interleaved across the 64 lanes of a wave, and a lane
+// that works alone would touch a cache line per element).
+constexpr size_t EPA_BYTES = sizeof(V3) * EPA_MAXV + sizeof(V3) * EPA_MAXF + sizeof(double) * EPA_MAXF +
+                             sizeof(int2) * EPA_MAXE + sizeof(I3) * EPA_MAXF;
+constexpr size_t CLIP_BYTES = 2 * sizeof(V3) * CLIP_MAX;
+constexpr size_t GROUP_WORK_BYTES = ((EPA_BYTES > CLIP_BYTES + MAXC * sizeof(V3) * 3 ? EPA_BYTES
+                                      : CLIP_BYTES + MAXC * sizeof(V3) * 3) + 255) / 256 * 256;
+RP_HD void epa_bind(EpaPoly& P, char* m) {
+    P.poly = (V3*)m;
+    m += sizeof(V3) * EPA_MAXV;
+    P.normals = (V3*)m;
+    m += sizeof(V3) * EPA_MAXF;
+    P.dist = (double*)m;
+    m += sizeof(double) * EPA_MAXF;
+    P.edges = (int2*)m;
+    m += sizeof(int2) * EPA_MAXE;
+    P.faces = (I3*)m;
+}
+
 // epa (epa.cpp:118-237): 0 = no convergence, 1 = converged, -1 = error.
 // G > 1: the group's lanes compute each support point together; the
-// polytope lives in lane 0 alone (its private arrays), which expands it and
-// hands the next direction and the status to the group. Every lane returns
+// polytope lives in lane 0 alone (in the group's working memory `work`),
+// which expands it and hands the next direction and the status to the group. Every lane returns
 // the status and, on convergence, lane 0's n_out / pen_out; the EPA error
 // bits are set in lane 0's err only.
 template <int G = 1, class View = HullView>
-RP_HD int epa(const View& h1, const View& h2, const Simplex& s, V3& n_out, double& pen_out, uint32_t& err) {
-    V3 poly[EPA_MAXV];
-    I3 faces[EPA_MAXF];
-    V3 normals[EPA_MAXF];
-    double dist[EPA_MAXF];
-    int2 edges[EPA_MAXE];
-    EpaPoly P;
-    P.poly = poly; P.faces = faces; P.normals = normals; P.dist = dist; P.edges = edges;
+RP_HD int epa(const View& h1, const View& h2, const Simplex& s, V3& n_out, double& pen_out, uint32_t& err,
+              char* work = nullptr) {
     if constexpr (G == 1) {
+        V3 poly[EPA_MAXV];
+        I3 faces[EPA_MAXF];
+        V3 normals[EPA_MAXF];
+        double dist[EPA_MAXF];
+        int2 edges[EPA_MAXE];
+        EpaPoly P;
+        P.poly = poly; P.faces = faces; P.normals = normals; P.dist = dist; P.edges = edges;
         if (!P.init(s, err)) return -1;
         for (int it = 0; it < 100; ++it) {
             V3 sp = support_md<G>(h1, h2, P.min_n, err);
@@ -568,6 +591,8 @@ RP_HD int epa(const View& h1, const View& h2, const Simplex& s, V3& n_out, doubl
         }
         return 0;
     } else {
+        EpaPoly P;
+        epa_bind(P, work);
         bool leader = (__lane_id() & (G - 1)) == 0;
         int st = 2;
         V3 dir = v3(0.0, 0.0, 0.0);
@@ -681,8 +706,9 @@ struct ContactPt { V3 p1, p2, n; };
 // of contacts written to `out` (at most MAXC). G > 1: the group's lanes find
 // the support points, the best faces and the best edge pair together (split
 // scans, first-maximum reductions); clipping and the contact points run on
-// lane 0 of the group alone, whose `out`, return value and error bits are
-// the results (the other lanes return 0).
+// lane 0 of the group alone, on the group's working memory `work` (its first
+// CLIP_BYTES), whose `out`, return value and error bits are the results (the
+// other lanes return 0).
 // get_face_with_most_fitting_normal over the group: lane k scores the
 // entries k, k + G, ... of the vertex's face list, then the group keeps the
 // first maximum (larger projection, ties to the earlier entry).
@@ -702,7 +728,8 @@ __device__ __forceinline__ int best_face_group(int sidx, const HullView& h, V3 n
 }
 
 template <int G = 1>
-RP_HD int manifold(const HullView& h1, const HullView& h2, V3 normal, ContactPt* out, uint32_t& err) {
+RP_HD int manifold(const HullView& h1, const HullView& h2, V3 normal, ContactPt* out, uint32_t& err,
+                   char* work = nullptr) {
     const double EPSILON = 0.0001;
     V3 inormal = inv3(normal);
     int s1, s2;
@@ -786,7 +813,10 @@ RP_HD int manifold(const HullView& h1, const HullView& h2, V3 normal, ContactPt*
     const HullView& rh = ref1 ? h1 : h2;
     const HullView& ih = ref1 ? h2 : h1;
     int rf = ref1 ? f1 : f2, inf = ref1 ? f2 : f1;
-    V3 bufA[CLIP_MAX], bufB[CLIP_MAX];
+    // clipping polygons: private arrays, or (G > 1) the group's working memory
+    V3 localA[G == 1 ? CLIP_MAX : 1], localB[G == 1 ? CLIP_MAX : 1];
+    V3* bufA = G == 1 ? localA : (V3*)work;
+    V3* bufB = G == 1 ? localB : (V3*)work + CLIP_MAX;
     int n = 0;
     for (int i = ih.fe_off[inf]; i < ih.fe_off[inf + 1]; ++i) {
         if (n < CLIP_MAX) bufA[n++] = ih.tv[ih.fe[i]]; else err |= ERR_CLIP_CAPACITY;
@@ -838,8 +868,10 @@ RP_HD int manifold(const HullView& h1, const HullView& h2, V3 normal, ContactPt*
 // drops the pair's contacts, epa.cpp:233-236, collider.cpp:549-551).
 // `flags` gets PAIR_EPA_FAIL and PAIR_GJK_CYCLE (a cycle exit of gjk).
 enum { PAIR_EPA_FAIL = 1, PAIR_GJK_CYCLE = 2 };
+// G > 1: lane-group working memory `work`; the results are lane 0's.
 template <int G = 1>
-RP_HD int pair_contacts(const HullView& h1, const HullView& h2, ContactPt* out, uint32_t& err, int& flags) {
+RP_HD int pair_contacts(const HullView& h1, const HullView& h2, ContactPt* out, uint32_t& err, int& flags,
+                        char* work = nullptr) {
     Simplex s;
     bool cycled = false;
     if (!gjk<G>(h1, h2, s, err, cycled)) {
@@ -847,9 +879,9 @@ RP_HD int pair_contacts(const HullView& h1, const HullView& h2, ContactPt* out, 
         return 0;
     }
     V3 n; double pen;
-    int r = epa<G>(h1, h2, s, n, pen, err);
+    int r = epa<G>(h1, h2, s, n, pen, err, work);
     if (r != 1) { flags |= PAIR_EPA_FAIL; return 0; }
-    return manifold<G>(h1, h2, n, out, err);
+    return manifold<G>(h1, h2, n, out, err, work);
 }
 
 // clipping_get_contact_manifold (clipping.cpp:343-371) for colliders: a

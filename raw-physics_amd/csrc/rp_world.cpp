@@ -353,6 +353,21 @@ struct DeviceImage {
     void release() {
         for (void* p : allocs) (void)hipFree(p);
         allocs.clear();
+        if (d.group_work) (void)hipFree(d.group_work);
+        d.group_work = nullptr;
+        d.group_work_slots = 0;
+    }
+    // Lane-group working memory for the current schedules (grown, never shrunk;
+    // the caller has synchronized the stream).
+    bool ensure_group_work() {
+        int64_t need = group_work_slots_needed(d);
+        if (need <= d.group_work_slots) return true;
+        if (d.group_work) (void)hipFree(d.group_work);
+        d.group_work = nullptr;
+        d.group_work_slots = 0;
+        RP_CHECK(hipMalloc(&d.group_work, (size_t)need * GROUP_WORK_BYTES));
+        d.group_work_slots = need;
+        return true;
     }
 };
 
@@ -486,6 +501,7 @@ static bool build_image(const HostWorld& hw, DeviceImage& im) {
     // stages them, else by the one-lane-per-pair manifold kernel
     d.manifold_group = d.narrow_group > 1 ? 0 : 1;
     d.manifold_blocks = narrow_blocks_for(d, 1, pcap);
+    if (!im.ensure_group_work()) return false;
     d.solve_lanes = 64;
 
     bool ok = true;
@@ -757,8 +773,23 @@ static hipError_t frame_graph(rp_world* w, const FrameParams& fp) {
     return hipSuccess;
 }
 
+// Every world launches on one process-wide stream. The runtime keeps, per
+// hardware queue, scratch memory sized for the largest kernel that queue has
+// run; worlds on streams of their own spread over several queues and their
+// retained scratch adds up until a later allocation fails
+// (HSA_STATUS_ERROR_OUT_OF_RESOURCES). Calls on different worlds therefore
+// run one after the other on the device.
+static hipStream_t shared_stream() {
+    static std::mutex mu;
+    static hipStream_t st = nullptr;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!st && hipStreamCreate(&st) != hipSuccess) st = nullptr;
+    return st;
+}
+
 static bool world_init(rp_world* w) {
-    if (hipStreamCreate(&w->stream) != hipSuccess) {
+    w->stream = shared_stream();
+    if (!w->stream) {
         set_msg("hipStreamCreate failed");
         return false;
     }
@@ -780,7 +811,6 @@ static void world_free(rp_world* w) {
         if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : w->timer.pool) (void)hipEventDestroy(e);
     w->im.release();
-    if (w->stream) (void)hipStreamDestroy(w->stream);
     delete w;
 }
 
@@ -1186,8 +1216,14 @@ uint32_t rp_world_adjoint_backward(rp_world* w, const double* dl_dstate, double*
 int32_t rp_world_set_narrow_group(rp_world* w, int32_t lanes_per_pair) {
     if (!narrow_group_supported(w->im.d, lanes_per_pair)) return -1;
     (void)hipStreamSynchronize(w->stream);
+    int old_g = w->im.d.narrow_group, old_b = w->im.d.narrow_blocks;
     w->im.d.narrow_group = lanes_per_pair;
     w->im.d.narrow_blocks = narrow_blocks_for(w->im.d, lanes_per_pair, w->im.pair_capacity);
+    if (!w->im.ensure_group_work()) {
+        w->im.d.narrow_group = old_g;
+        w->im.d.narrow_blocks = old_b;
+        return -1;
+    }
     ++w->sched_gen;
     return 0;
 }
@@ -1197,8 +1233,14 @@ int32_t rp_world_narrow_group(const rp_world* w) { return w->im.d.narrow_group; 
 int32_t rp_world_set_manifold_group(rp_world* w, int32_t lanes_per_pair) {
     if (lanes_per_pair != 0 && !narrow_group_supported(w->im.d, lanes_per_pair)) return -1;
     (void)hipStreamSynchronize(w->stream);
+    int old_g = w->im.d.manifold_group, old_b = w->im.d.manifold_blocks;
     w->im.d.manifold_group = lanes_per_pair;
     w->im.d.manifold_blocks = narrow_blocks_for(w->im.d, lanes_per_pair ? lanes_per_pair : 1, w->im.pair_capacity);
+    if (!w->im.ensure_group_work()) {
+        w->im.d.manifold_group = old_g;
+        w->im.d.manifold_blocks = old_b;
+        return -1;
+    }
     ++w->sched_gen;
     return 0;
 }
