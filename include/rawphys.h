@@ -317,6 +317,18 @@ int32_t rp_world_narrow_group(const rp_world* w);
 int32_t rp_world_set_manifold_group(rp_world* w, int32_t lanes_per_pair);
 int32_t rp_world_manifold_group(const rp_world* w);
 
+/* Concurrent views: graph replay cuts the world into up to 4 scene ranges
+ * of about equal body counts and replays each on a stream of its own, so
+ * that one range's latency-bound kernels overlap another's (scenes never
+ * interact, so results are identical). Used with the grouped narrow-phase
+ * schedules (narrow group > 1, manifold group 0, 4, 8 or 16) in worlds
+ * without sphere or compound pairs. views: 0 = automatic (3 from 64
+ * scenes: with the runtime's default of four hardware queues per process,
+ * a fourth view's stream shares a queue), 1 = none, 2..4. rp_world_views:
+ * the number the last captured frame used. Returns 0 or -1. */
+int32_t rp_world_set_views(rp_world* w, int32_t views);
+int32_t rp_world_views(const rp_world* w);
+
 /* Solve schedule: lanes serving one scene in the solve kernel (64, 32 or 16,
  * i.e. 1, 2 or 4 scenes per wave). Identical results. Returns 0 or -1. */
 int32_t rp_world_set_solve_lanes(rp_world* w, int32_t lanes_per_scene);

@@ -67,6 +67,8 @@ def library():
         "rp_world_set_solve_lanes": (i32, [_P, i32]),
         "rp_world_solve_lanes": (i32, [_P]),
         "rp_world_set_graphs": (None, [_P, i32]),
+        "rp_world_set_views": (i32, [_P, i32]),
+        "rp_world_views": (i32, [_P]),
         "rp_world_kernel_times": (None, [_P, _P, _P]),
     }
     for name, (res, args) in sig.items():
@@ -247,6 +249,15 @@ class World:
         self.set_manifold_group(m)
         self.set_solve_lanes(min(out["solve"], key=out["solve"].get))
         return out
+
+    def set_views(self, views: int):
+        """Concurrent scene-range views for graph replay: 0 = automatic, 1..4."""
+        if self.lib.rp_world_set_views(self.h, views) != 0:
+            raise ValueError(f"views must be 0..4, not {views}")
+
+    @property
+    def views(self) -> int:
+        return int(self.lib.rp_world_views(self.h))
 
     def set_graphs(self, on: bool):
         """Replay each frame as a captured HIP graph (default) or launch eagerly."""

@@ -10,8 +10,13 @@ enum { STAT_EPA_FAIL = 0, STAT_GJK_CYCLE = 1, STAT_ADJ_MISMATCH = 2, STAT_COUNT 
 enum { CNT_MANIFOLD = 0, CNT_COUNT = 4 };
 
 // All device pointers of one world. Passed by value to every kernel.
+// A world or a view of it: views (scene ranges of one world, launched on
+// streams of their own) shift the scene-, pair- and queue-indexed pointers to
+// their range and keep body, hull, pair-slot and item indices absolute;
+// body_base is the first body of the range for the kernels over bodies.
 struct DevWorld {
     int ns, nb, n_xf, narrow_blocks, n_ext;
+    int body_base;
     int narrow_group;               // lanes per pair in GJK + EPA: 1 (lane per pair) or 4, 8, 16
     int manifold_group;             // lanes per pair in the contact manifold: 1, 4, 8, 16
     int manifold_blocks;

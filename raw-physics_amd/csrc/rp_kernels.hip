@@ -30,6 +30,7 @@ namespace rp {
 __global__ void k_frame_begin(DevWorld w, double gravity) {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= w.nb) return;
+    b += w.body_base;
     Body& e = w.body[b];
     // Caller pattern (stack.cpp:93-96) and force / torque sums over its one
     // force (physics_util.cpp:5-23).
@@ -178,6 +179,7 @@ __global__ void k_integrate(DevWorld w, double h) {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b == 0) w.counters[CNT_MANIFOLD] = 0;  // the substep's manifold queue starts empty
     if (b >= w.nb) return;
+    b += w.body_base;
     Body e = w.body[b];
     integrate(e, h);
     w.body[b].x = e.x;
@@ -196,6 +198,7 @@ __global__ void k_integrate(DevWorld w, double h) {
 __global__ void k_keep_pose(DevWorld w) {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= w.nb) return;
+    b += w.body_base;
     const Body& e = w.body[b];
     double* p = w.pose + 7 * (size_t)b;
     p[0] = e.x.x; p[1] = e.x.y; p[2] = e.x.z;

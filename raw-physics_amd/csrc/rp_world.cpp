@@ -16,6 +16,13 @@
 
 namespace rp {
 
+// Scene-range views of a world replayed concurrently, one stream each. With
+// the runtime's default four hardware queues per process, three views ran
+// the batched workload fastest on MI355X (3.47 ms/frame, against 3.70 for
+// two, 4.45 for one; a fourth stream shared a queue and took 5.87).
+constexpr int kMaxViews = 4;
+constexpr int kAutoViews = 3;
+
 static thread_local uint32_t g_last_err = 0;
 static thread_local std::string g_last_msg;
 
@@ -323,6 +330,8 @@ struct DeviceImage {
     std::vector<BigScene> big;  // large scenes (device buffers in allocs)
     std::vector<void*> allocs;
     std::vector<int32_t> body_tv_off;  // host copy for inspection
+    std::vector<int32_t> pair_base;    // host copy of scene_pair_base
+    std::vector<int32_t> body_xf_off;  // [nb+1] first transform-list entry of each body
     int64_t pair_capacity = 0;
     bool ok = false;
 
@@ -359,8 +368,8 @@ struct DeviceImage {
     }
     // Lane-group working memory for the current schedules (grown, never shrunk;
     // the caller has synchronized the stream).
-    bool ensure_group_work() {
-        int64_t need = group_work_slots_needed(d);
+    bool ensure_group_work(int64_t at_least = 0) {
+        int64_t need = std::max(group_work_slots_needed(d), at_least);
         if (need <= d.group_work_slots) return true;
         if (d.group_work) (void)hipFree(d.group_work);
         d.group_work = nullptr;
@@ -414,9 +423,11 @@ static bool build_image(const HostWorld& hw, DeviceImage& im) {
     std::vector<float> col_radius;
     int64_t ntv = 0;
     im.body_tv_off.resize(d.nb);
+    im.body_xf_off.assign(d.nb + 1, 0);
     for (int b = 0; b < d.nb; ++b) {
         int c0 = hw.body_col_off[b], c1 = hw.body_col_off[b + 1];
         im.body_tv_off[b] = (int32_t)ntv;
+        im.body_xf_off[b] = (int32_t)xf.size();
         if (c1 - c0 != 1 || hw.cols[c0].type != COL_HULL) body_general[b] = 1;
         for (int c = c0; c < c1; ++c) {
             const HostCollider& hc = hw.cols[c];
@@ -431,6 +442,7 @@ static bool build_image(const HostWorld& hw, DeviceImage& im) {
         }
     }
     d.n_xf = (int)xf.size();
+    im.body_xf_off[d.nb] = d.n_xf;
     d.has_general = 0;
     for (int32_t g : body_general) d.has_general |= g;
     // pair and constraint capacities per scene; large scenes get a bounded
@@ -493,6 +505,7 @@ static bool build_image(const HostWorld& hw, DeviceImage& im) {
     for (const HostHull& h : hw.hulls) max_nv = std::max(max_nv, (int)h.v.size());
     d.stage_cap = 2 * max_nv;
     im.pair_capacity = pcap;
+    im.pair_base = pair_base;
     d.narrow_group = 1;
     for (int g : {8, 16, 4})
         if (narrow_group_supported(d, g)) { d.narrow_group = g; break; }
@@ -540,12 +553,12 @@ static bool build_image(const HostWorld& hw, DeviceImage& im) {
     ok &= im.alloc(d.pair_count, d.ns);
     ok &= im.alloc(d.live_local, (size_t)pcap);
     ok &= im.alloc(d.live_count, d.ns);
-    ok &= im.alloc(d.live_prefix, d.ns + 1);
+    ok &= im.alloc(d.live_prefix, d.ns + 1 + kMaxViews);  // a view's prefix ends one slot past its scenes
     ok &= im.alloc(d.live_global, (size_t)pcap);
     ok &= im.alloc(d.ncontact, (size_t)pcap);
     ok &= im.alloc(d.pair_normal, (size_t)pcap);
     ok &= im.alloc(d.mlist, (size_t)pcap);
-    ok &= im.alloc(d.counters, CNT_COUNT);
+    ok &= im.alloc(d.counters, CNT_COUNT * kMaxViews);
     ok &= im.alloc(d.contacts, (size_t)pcap * MAXC);
     ok &= im.upload(d.ext, hw.ext);
     ok &= im.alloc(d.ext_lambda, 3 * hw.ext.size());
@@ -710,13 +723,20 @@ struct rp_world {
     KernelTimer timer;              // profiling mode
     double kc_ms[KC_COUNT] = {0};
     uint64_t kc_launches[KC_COUNT] = {0};
-    // one frame's launches captured as a HIP graph and replayed per frame
+    // one frame's launches captured as a HIP graph per view and replayed per
+    // frame; views are scene ranges of the world on streams of their own
     bool use_graphs = true;
-    hipGraphExec_t frame_exec = nullptr;
+    int want_views = 0;  // 0 = automatic
+    int nviews = 0;
+    DevWorld view[kMaxViews];
+    std::vector<BigScene> view_big[kMaxViews];
+    hipStream_t view_stream[kMaxViews] = {};
+    hipGraphExec_t view_exec[kMaxViews] = {};
+    hipEvent_t fork_ev = nullptr, join_ev[kMaxViews] = {};
     FrameParams graph_fp;
-    uint64_t sched_gen = 1, graph_gen = 0;  // schedule changes invalidate the captured graph
-    // bounded queue: at most two chunks of frames in flight on the stream
-    hipEvent_t chunk_done[2] = {nullptr, nullptr};
+    uint64_t sched_gen = 1, graph_gen = 0;  // schedule changes invalidate the captured graphs
+    // bounded queue: at most two chunks of frames in flight on each stream
+    hipEvent_t chunk_done[2][kMaxViews] = {};
     // differentiable runs: per-frame body checkpoints of the last recorded
     // forward pass and the buffers of the backward sweep
     Body* ckpt = nullptr;
@@ -747,44 +767,137 @@ static bool same_params(const FrameParams& a, const FrameParams& b) {
            a.collisions == b.collisions && a.forces_given == b.forces_given && a.keep_pose == b.keep_pose;
 }
 
-// The graph of one frame for these parameters and the current schedules,
-// captured once and reused while they do not change.
-static hipError_t frame_graph(rp_world* w, const FrameParams& fp) {
-    const DevWorld& d = w->im.d;
-    if (w->frame_exec && same_params(w->graph_fp, fp) && w->graph_gen == w->sched_gen) return hipSuccess;
-    if (w->frame_exec) {
-        (void)hipGraphExecDestroy(w->frame_exec);
-        w->frame_exec = nullptr;
-    }
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal);
-    if (e != hipSuccess) return e;
-    hipError_t le = launch_frame(d, fp, w->stream, nullptr, w->im.big.data(), (int)w->im.big.size());
-    e = hipStreamEndCapture(w->stream, &g);
-    if (le != hipSuccess) e = le;
-    if (e == hipSuccess) e = hipGraphInstantiate(&w->frame_exec, g, nullptr, nullptr, 0);
-    if (g) (void)hipGraphDestroy(g);
-    if (e != hipSuccess) {
-        w->frame_exec = nullptr;
-        return e;
-    }
-    w->graph_fp = fp;
-    w->graph_gen = w->sched_gen;
-    return hipSuccess;
-}
-
 // Every world launches on one process-wide stream. The runtime keeps, per
 // hardware queue, scratch memory sized for the largest kernel that queue has
 // run; worlds on streams of their own spread over several queues and their
 // retained scratch adds up until a later allocation fails
 // (HSA_STATUS_ERROR_OUT_OF_RESOURCES). Calls on different worlds therefore
-// run one after the other on the device.
-static hipStream_t shared_stream() {
+// run one after the other on the device. A world's views use streams 1..3
+// of the same pool (the kernels they run keep no large private memory: see
+// build_views).
+static hipStream_t pool_stream(int k) {
     static std::mutex mu;
-    static hipStream_t st = nullptr;
+    static hipStream_t st[kMaxViews] = {};
     std::lock_guard<std::mutex> lock(mu);
-    if (!st && hipStreamCreate(&st) != hipSuccess) st = nullptr;
-    return st;
+    if (!st[k] && hipStreamCreate(&st[k]) != hipSuccess) st[k] = nullptr;
+    return st[k];
+}
+static hipStream_t shared_stream() { return pool_stream(0); }
+
+// Views for graph replay: with a grouped narrow schedule whose kernels keep
+// no large private memory (no one-lane kernels, no sphere or compound pairs)
+// and enough scenes, the world is cut into up to kMaxViews scene ranges of
+// about equal body counts, each replayed on a stream of its own. A scene's
+// work never touches another's, so the views only run concurrently: while
+// one view's latency-bound solve runs, another's narrow phase fills the
+// SIMDs. Otherwise one view: the whole world on the main stream. Worlds of
+// 64 scenes or more get kAutoViews views unless set otherwise.
+static bool build_views(rp_world* w) {
+    const DevWorld& d = w->im.d;
+    DeviceImage& im = w->im;
+    int K = 1;
+    if (d.narrow_group > 1 && d.manifold_group != 1 && !d.has_general) {
+        int want = w->want_views > 0 ? w->want_views : (d.ns >= 64 ? kAutoViews : 1);
+        K = std::max(1, std::min({want, kMaxViews, d.ns}));
+    }
+    for (int k = 0; k < K; ++k) {
+        w->view_stream[k] = k == 0 ? w->stream : pool_stream(k);
+        if (!w->view_stream[k]) { set_msg("hipStreamCreate failed"); return false; }
+    }
+    if (K == 1) {
+        w->view[0] = d;
+        w->view_big[0] = im.big;
+        w->nviews = 1;
+        return true;
+    }
+    // scene cuts at about equal body counts
+    std::vector<int> cut(K + 1, 0);
+    cut[K] = d.ns;
+    for (int k = 1; k < K; ++k) {
+        int64_t target = (int64_t)d.nb * k / K;
+        int s = cut[k - 1];
+        while (s < d.ns && w->hw.scene_body_off[s] < target) ++s;
+        cut[k] = std::max(s, cut[k - 1]);
+    }
+    int64_t slots = 0;
+    for (int k = 0; k < K; ++k) {
+        int s0 = cut[k], s1 = cut[k + 1];
+        DevWorld v = d;
+        int b0 = w->hw.scene_body_off[s0], b1 = w->hw.scene_body_off[s1];
+        int64_t P0 = im.pair_base[s0], P1 = im.pair_base[s1];
+        v.ns = s1 - s0;
+        v.body_base = b0;
+        v.nb = b1 - b0;
+        v.n_ext = w->hw.scene_ext_off[s1] - w->hw.scene_ext_off[s0];
+        v.scene_body_off += s0; v.scene_pair_base += s0; v.scene_ext_off += s0; v.scene_item_base += s0;
+        v.pair_count += s0; v.live_count += s0; v.live_prefix += s0 + k;
+        v.scene_nlevels += s0; v.scene_big += s0; v.scene_err += s0;
+        v.item_lstart += s0; v.item_cursor += s0;
+        v.live_global += P0; v.ncontact += P0; v.pair_normal += P0; v.mlist += P0;
+        v.contacts += P0 * MAXC;
+        v.counters += CNT_COUNT * k;
+        v.xf += im.body_xf_off[b0];
+        v.n_xf = im.body_xf_off[b1] - im.body_xf_off[b0];
+        v.narrow_blocks = std::min(narrow_blocks_for(v, v.narrow_group, P1 - P0), std::max(1, d.narrow_blocks / K));
+        v.manifold_blocks = std::min(narrow_blocks_for(v, v.manifold_group ? v.manifold_group : 1, P1 - P0),
+                                     std::max(1, d.manifold_blocks / K));
+        v.group_work = nullptr;
+        w->view[k] = v;
+        w->view_big[k].clear();
+        for (const BigScene& bs : im.big)
+            if (bs.s >= s0 && bs.s < s1) {
+                BigScene c = bs;
+                c.s -= s0;
+                w->view_big[k].push_back(c);
+            }
+        slots += group_work_slots_needed(v);
+    }
+    if (!im.ensure_group_work(slots)) return false;
+    int64_t at = 0;
+    for (int k = 0; k < K; ++k) {
+        w->view[k].group_work = im.d.group_work + (size_t)at * GROUP_WORK_BYTES;
+        w->view[k].group_work_slots = group_work_slots_needed(w->view[k]);
+        at += w->view[k].group_work_slots;
+    }
+    w->nviews = K;
+    return true;
+}
+
+static void drop_graphs(rp_world* w) {
+    for (hipGraphExec_t& e : w->view_exec)
+        if (e) {
+            (void)hipGraphExecDestroy(e);
+            e = nullptr;
+        }
+}
+
+// The graphs of one frame for these parameters and the current schedules,
+// captured once per view and reused while they do not change.
+static hipError_t frame_graph(rp_world* w, const FrameParams& fp) {
+    if (w->nviews > 0 && w->view_exec[0] && same_params(w->graph_fp, fp) && w->graph_gen == w->sched_gen)
+        return hipSuccess;
+    (void)hipStreamSynchronize(w->stream);
+    for (int k = 1; k < w->nviews; ++k) (void)hipStreamSynchronize(w->view_stream[k]);
+    drop_graphs(w);
+    if (!build_views(w)) return hipErrorOutOfMemory;
+    for (int k = 0; k < w->nviews; ++k) {
+        hipStream_t st = w->view_stream[k];
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return e;
+        hipError_t le = launch_frame(w->view[k], fp, st, nullptr, w->view_big[k].data(), (int)w->view_big[k].size());
+        e = hipStreamEndCapture(st, &g);
+        if (le != hipSuccess) e = le;
+        if (e == hipSuccess) e = hipGraphInstantiate(&w->view_exec[k], g, nullptr, nullptr, 0);
+        if (g) (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            drop_graphs(w);
+            return e;
+        }
+    }
+    w->graph_fp = fp;
+    w->graph_gen = w->sched_gen;
+    return hipSuccess;
 }
 
 static bool world_init(rp_world* w) {
@@ -793,11 +906,15 @@ static bool world_init(rp_world* w) {
         set_msg("hipStreamCreate failed");
         return false;
     }
-    for (hipEvent_t& e : w->chunk_done) {
-        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
-            set_msg("hipEventCreate failed");
-            return false;
-        }
+    bool ok = hipEventCreateWithFlags(&w->fork_ev, hipEventDisableTiming) == hipSuccess;
+    for (int k = 0; k < kMaxViews; ++k) {
+        ok &= hipEventCreateWithFlags(&w->join_ev[k], hipEventDisableTiming) == hipSuccess;
+        for (int c = 0; c < 2; ++c)
+            ok &= hipEventCreateWithFlags(&w->chunk_done[c][k], hipEventDisableTiming) == hipSuccess;
+    }
+    if (!ok) {
+        set_msg("hipEventCreate failed");
+        return false;
     }
     return build_image(w->hw, w->im);
 }
@@ -805,10 +922,15 @@ static bool world_init(rp_world* w) {
 static void world_free(rp_world* w) {
     if (!w) return;
     if (w->stream) (void)hipStreamSynchronize(w->stream);
-    if (w->frame_exec) (void)hipGraphExecDestroy(w->frame_exec);
+    for (int k = 1; k < w->nviews; ++k) (void)hipStreamSynchronize(w->view_stream[k]);
+    drop_graphs(w);
     adjoint_free(w);
-    for (hipEvent_t e : w->chunk_done)
-        if (e) (void)hipEventDestroy(e);
+    for (int k = 0; k < kMaxViews; ++k) {
+        if (w->join_ev[k]) (void)hipEventDestroy(w->join_ev[k]);
+        for (int c = 0; c < 2; ++c)
+            if (w->chunk_done[c][k]) (void)hipEventDestroy(w->chunk_done[c][k]);
+    }
+    if (w->fork_ev) (void)hipEventDestroy(w->fork_ev);
     for (hipEvent_t e : w->timer.pool) (void)hipEventDestroy(e);
     w->im.release();
     delete w;
@@ -817,21 +939,33 @@ static void world_free(rp_world* w) {
 // Keeps the stream's queue bounded: after each chunk of frames an event is
 // recorded, and the host waits for the chunk before the previous one, so at
 // most two chunks are queued while the device always has one to run.
-static hipError_t chunk_mark(rp_world* w, uint32_t f) {
+static hipError_t chunk_mark(rp_world* w, uint32_t f, int view, hipStream_t st) {
     if ((f + 1) % kFramesPerChunk != 0) return hipSuccess;
     uint32_t k = (f + 1) / kFramesPerChunk;
     hipError_t e = hipSuccess;
-    if (k >= 2) e = hipEventSynchronize(w->chunk_done[k & 1]);
-    if (e == hipSuccess) e = hipEventRecord(w->chunk_done[k & 1], w->stream);
+    if (k >= 2) e = hipEventSynchronize(w->chunk_done[k & 1][view]);
+    if (e == hipSuccess) e = hipEventRecord(w->chunk_done[k & 1][view], st);
     return e;
 }
 
 static uint32_t world_run(rp_world* w, const FrameParams& fp, uint32_t frames, bool sync) {
     if (w->use_graphs && !w->timer.on && frames > 0) {
         hipError_t e = frame_graph(w, fp);
+        // the views start after the main stream's earlier work and the main
+        // stream continues after theirs
+        if (e == hipSuccess && w->nviews > 1) {
+            e = hipEventRecord(w->fork_ev, w->stream);
+            for (int k = 1; e == hipSuccess && k < w->nviews; ++k) e = hipStreamWaitEvent(w->view_stream[k], w->fork_ev, 0);
+        }
         for (uint32_t f = 0; e == hipSuccess && f < frames; ++f) {
-            e = hipGraphLaunch(w->frame_exec, w->stream);
-            if (e == hipSuccess) e = chunk_mark(w, f);
+            for (int k = 0; e == hipSuccess && k < w->nviews; ++k) {
+                e = hipGraphLaunch(w->view_exec[k], w->view_stream[k]);
+                if (e == hipSuccess) e = chunk_mark(w, f, k, w->view_stream[k]);
+            }
+        }
+        for (int k = 1; e == hipSuccess && k < w->nviews; ++k) {
+            e = hipEventRecord(w->join_ev[k], w->view_stream[k]);
+            if (e == hipSuccess) e = hipStreamWaitEvent(w->stream, w->join_ev[k], 0);
         }
         if (e != hipSuccess) {
             set_msg(std::string("frame graph: ") + hipGetErrorString(e));
@@ -842,7 +976,7 @@ static uint32_t world_run(rp_world* w, const FrameParams& fp, uint32_t frames, b
     for (uint32_t f = 0; f < frames; ++f) {
         w->timer.used = 0;
         hipError_t e = launch_frame(w->im.d, fp, w->stream, &w->timer, w->im.big.data(), (int)w->im.big.size());
-        if (e == hipSuccess && !w->timer.on) e = chunk_mark(w, f);
+        if (e == hipSuccess && !w->timer.on) e = chunk_mark(w, f, 0, w->stream);
         if (e != hipSuccess) {
             set_msg(std::string("kernel launch: ") + hipGetErrorString(e));
             return RP_ERR_RUNTIME;
@@ -1256,6 +1390,15 @@ int32_t rp_world_set_solve_lanes(rp_world* w, int32_t lanes_per_scene) {
 }
 
 int32_t rp_world_solve_lanes(const rp_world* w) { return w->im.d.solve_lanes; }
+
+int32_t rp_world_set_views(rp_world* w, int32_t views) {
+    if (views < 0 || views > kMaxViews) return -1;
+    w->want_views = views;
+    ++w->sched_gen;
+    return 0;
+}
+
+int32_t rp_world_views(const rp_world* w) { return w->nviews; }
 
 void rp_world_set_graphs(rp_world* w, int32_t on) {
     (void)hipStreamSynchronize(w->stream);
