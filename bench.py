@@ -184,6 +184,40 @@ def large_run(pkg, sc, n: int, frames: int):
     return out
 
 
+def uneven_run(pkg, sc, n_scenes: int, frames: int, tune_from: int):
+    """Batched uneven workload: n_scenes scenes of 4-256 bodies with 8/42/162-
+    vertex hulls, 30% carrying joint chains; the autotuner picks the
+    schedules on frames from a mid-run state; body-substeps/s over `frames`
+    frames from the start (graph replay)."""
+    t0 = time.perf_counter()
+    world = sc.batched_uneven(n_scenes)
+    t_build = time.perf_counter() - t0
+    gw = pkg.World(world)
+    init = gw.state()
+    gw.step(tune_from)
+    tune = gw.autotune(frames=4)
+    gw.set_state(init)
+    gw.step(2)
+    gw.set_state(init)
+    gw.synchronize()
+    t0 = time.perf_counter()
+    gw.step(frames, sync=False)
+    gw.synchronize()
+    el = time.perf_counter() - t0
+    dyn = world.dynamic_bodies()
+    errs = gw.errors()
+    out = {"workload": f"batched uneven: {n_scenes} scenes, 4-256 bodies (log-uniform), 8/42/162-vertex hulls, "
+                       f"30% with joint chains, {frames} frames",
+           "scenes": n_scenes, "dynamic_bodies": dyn, "constraints": len(world.constraints), "frames": frames,
+           "value": dyn * 20 * frames / el, "unit": "body-substeps/s", "ms_per_frame": 1e3 * el / frames,
+           "schedule": {"narrow": gw.narrow_group, "manifold": gw.manifold_group, "solve": gw.solve_lanes,
+                        "views": gw.views},
+           "autotune_ms": tune, "scene_build_s": t_build, "scenes_with_error_words": int((errs != 0).sum())}
+    gw.close()
+    log("uneven: " + json.dumps(out))
+    return out
+
+
 def copy_calibration(nbytes: int):
     """One hipMemcpy device to device of `nbytes` (the runtime's copy kernel
     reads and writes nbytes each)."""
@@ -227,6 +261,9 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurement")
     ap.add_argument("--adjoint-scenes", type=int, default=2048)
     ap.add_argument("--adjoint-frames", type=int, default=120)
+    ap.add_argument("--no-uneven", action="store_true", help="skip the batched uneven measurement")
+    ap.add_argument("--uneven-scenes", type=int, default=1024)
+    ap.add_argument("--uneven-frames", type=int, default=600)
     ap.add_argument("--large-bodies", type=int, default=0, help="also time one large scene of this many bodies")
     ap.add_argument("--large-frames", type=int, default=30)
     ap.add_argument("--save-state", metavar="NPZ", help="after the timed frames, save the batch's body state")
@@ -315,6 +352,7 @@ def main():
 
     adjoint = None if args.no_adjoint else adjoint_run(pkg, sc, args.adjoint_scenes, args.adjoint_frames)
     large = None if not args.large_bodies else large_run(pkg, sc, args.large_bodies, args.large_frames)
+    uneven = None if args.no_uneven else uneven_run(pkg, sc, args.uneven_scenes, args.uneven_frames, args.tune_from)
 
     plain = None
     if not args.no_plain:
@@ -406,6 +444,7 @@ def main():
                   "ratio_batched_over_plain": (value / plain) if plain else None},
         "adjoint": adjoint,
         "large_scene": large,
+        "uneven": uneven,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,

@@ -89,4 +89,30 @@ __device__ __forceinline__ void velocity_update_body(Body& g, double h) {
     g.pv = e.pv; g.pw = e.pw; g.v = e.v; g.w = e.w;
 }
 
+// Appends live pair g to the small-hull or the large-hull list (one atomic
+// per wave and list; the lists' order does not matter). All lanes of the
+// wave call it; `valid` marks the lanes holding a pair.
+__device__ __forceinline__ void classify_pair(const DevWorld& w, bool valid, int g, int b1, int b2) {
+    bool big = false;
+    if (valid) {
+        int h1 = w.body_hull[b1], h2 = w.body_hull[b2];
+        big = (h1 >= 0 && w.hull_nv[h1] > w.small_nv) || (h2 >= 0 && w.hull_nv[h2] > w.small_nv);
+    }
+    unsigned long long mb = __ballot(valid && big), ms = __ballot(valid && !big);
+    int lane = lane_id();
+    unsigned long long below = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
+    int base_b = 0, base_s = 0;
+    int leader = __ffsll((long long)(mb | ms)) - 1;
+    if (lane == leader) {
+        if (mb) base_b = atomicAdd(&w.counters[CNT_BIG], __popcll(mb));
+        if (ms) base_s = atomicAdd(&w.counters[CNT_SMALL], __popcll(ms));
+    }
+    if (leader < 0) return;
+    base_b = __shfl(base_b, leader);
+    base_s = __shfl(base_s, leader);
+    if (!valid) return;
+    if (big) w.pair_lists[w.list_stride + base_b + __popcll(mb & below)] = g;
+    else w.pair_lists[base_s + __popcll(ms & below)] = g;
+}
+
 }  // namespace rp

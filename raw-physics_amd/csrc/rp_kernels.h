@@ -7,7 +7,10 @@
 namespace rp {
 
 enum { STAT_EPA_FAIL = 0, STAT_GJK_CYCLE = 1, STAT_ADJ_MISMATCH = 2, STAT_COUNT = 4 };
-enum { CNT_MANIFOLD = 0, CNT_COUNT = 4 };
+enum { CNT_MANIFOLD = 0, CNT_SMALL = 1, CNT_BIG = 2, CNT_COUNT = 4 };
+// Hulls of up to this many vertices are "small": the live pairs of two small
+// hulls are staged with a per-pair LDS slice sized for them.
+constexpr int SMALL_HULL_NV = 64;
 
 // All device pointers of one world. Passed by value to every kernel.
 // A world or a view of it: views (scene ranges of one world, launched on
@@ -21,6 +24,8 @@ struct DevWorld {
     int manifold_group;             // lanes per pair in the contact manifold: 1, 4, 8, 16
     int manifold_blocks;
     int stage_cap;                  // LDS vertices per pair: 2 x the largest hull
+    int small_nv;                   // > 0: pairs of two hulls of <= small_nv vertices go to their own list
+    int stage_small;                // LDS vertices per pair of that list: 2 x small_nv
     int solve_lanes;                // lanes per scene in the solve: 64, 32 or 16
     // scenes
     const int32_t* scene_body_off;  // [ns+1]
@@ -62,6 +67,8 @@ struct DevWorld {
     int32_t* ncontact;              // [pair capacity]
     V3* pair_normal;                // [pair capacity] EPA normal of the pairs queued for the manifold
     int32_t* mlist;                 // [pair capacity] manifold queue: live-pair slots
+    int32_t* pair_lists;            // [2][pair capacity] live pairs of small hulls, of large hulls (by frame)
+    int64_t list_stride;            // offset of the large-hull list
     int32_t* counters;              // [CNT_COUNT] queue lengths of the current substep
     char* group_work;               // [group_work_slots][GROUP_WORK_BYTES] lane-group working memory
     int64_t group_work_slots;       // >= blocks x groups per block of the grouped narrow / manifold kernels

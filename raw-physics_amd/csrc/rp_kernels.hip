@@ -138,6 +138,10 @@ __global__ void k_islands(DevWorld w, double dt) {
 __global__ void k_scan_live(DevWorld w) {
     __shared__ int part[1024];
     int t = threadIdx.x, nt = blockDim.x;
+    if (t == 0) {  // the size-class lists of this frame start empty
+        w.counters[CNT_SMALL] = 0;
+        w.counters[CNT_BIG] = 0;
+    }
     int ns = w.ns;
     int chunk = (ns + nt - 1) / nt;
     int lo = min(ns, t * chunk), hi = min(ns, lo + chunk);
@@ -165,12 +169,20 @@ __global__ void k_scatter_live(DevWorld w) {
     int base = w.scene_pair_base[s];
     int g0 = w.live_prefix[s], n = w.live_count[s];
     int b0 = w.scene_body_off[s];
-    for (int m = threadIdx.x; m < n; m += blockDim.x) {
-        int slot = base + w.live_local[base + m];
-        w.live_global[g0 + m] = make_int2(s, slot);
-        int2 p = w.pairs[slot];
-        w.need[b0 + p.x] = 1;  // same value from every writer
-        w.need[b0 + p.y] = 1;
+    for (int m0 = 0; m0 < n; m0 += blockDim.x) {
+        int m = m0 + threadIdx.x;
+        bool valid = m < n;
+        int b1 = 0, b2 = 0;
+        if (valid) {
+            int slot = base + w.live_local[base + m];
+            w.live_global[g0 + m] = make_int2(s, slot);
+            int2 p = w.pairs[slot];
+            b1 = b0 + p.x;
+            b2 = b0 + p.y;
+            w.need[b1] = 1;  // same value from every writer
+            w.need[b2] = 1;
+        }
+        if (w.small_nv > 0) classify_pair(w, valid, g0 + m, b1, b2);
     }
 }
 
@@ -343,14 +355,18 @@ __device__ __forceinline__ void pair_views(const DevWorld& w, int g, V3* st, int
 
 // kManifold: the contact manifold of a converged pair is built here too, on
 // the same staged hulls, and its contacts written (no manifold queue).
+// list / count: visit the pairs of a size-class list (LDS slices of `stage`
+// vertices per pair), or (list null) every live pair with slices of
+// stage_cap vertices.
 template <int G, bool kManifold>
-__device__ __forceinline__ void gjk_epa_pairs(DevWorld& w) {
+__device__ __forceinline__ void gjk_epa_pairs(DevWorld& w, const int32_t* list, const int32_t* count, int stage) {
     extern __shared__ V3 rp_stage[];
     constexpr int NG = 64 / G;
     int grp = threadIdx.x / G, gl = threadIdx.x & (G - 1);
-    V3* st = rp_stage + grp * w.stage_cap;
-    int total = w.live_prefix[w.ns];
-    for (int g = blockIdx.x * NG + grp; g < total; g += gridDim.x * NG) {
+    V3* st = rp_stage + grp * stage;
+    int total = list ? *count : w.live_prefix[w.ns];
+    for (int k = blockIdx.x * NG + grp; k < total; k += gridDim.x * NG) {
+        int g = list ? list[k] : k;
         int s, b1, b2;
         pair_bodies(w, g, s, b1, b2);
         if (general_pair(w, b1, b2)) continue;  // k_narrow_general (uniform per group)
@@ -397,13 +413,14 @@ __device__ __forceinline__ void gjk_epa_pairs(DevWorld& w) {
 
 // Lane groups (G = 4, 8, 16) at two waves per SIMD; one lane per pair (G = 1).
 template <int G, bool kManifold>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_gjk_epa(DevWorld w) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+k_gjk_epa(DevWorld w, const int32_t* list, const int32_t* count, int stage) {
     static_assert(G > 1, "lane groups");
-    gjk_epa_pairs<G, kManifold>(w);
+    gjk_epa_pairs<G, kManifold>(w, list, count, stage);
 }
 template <bool kManifold>
 __global__ void __launch_bounds__(64) k_gjk_epa_lane(DevWorld w) {
-    gjk_epa_pairs<1, kManifold>(w);
+    gjk_epa_pairs<1, kManifold>(w, nullptr, nullptr, 0);
 }
 
 template <int G>
@@ -718,18 +735,32 @@ hipError_t launch_frame(const DevWorld& w, const FrameParams& fp, hipStream_t st
             t.begin(KC_NARROW, st);
             size_t lds = w.narrow_group > 1 ? (size_t)(64 / w.narrow_group) * w.stage_cap * sizeof(V3) : 0;
             dim3 nb(w.narrow_blocks), t64(64);
-            if (fused) {
+            if (fused && w.small_nv > 0 && w.narrow_group > 1) {
+                // two size classes: small-hull pairs with small LDS slices on the
+                // narrow schedule, the others on the schedule with the fewest
+                // groups per block that fits
+                const int32_t* small = w.pair_lists;
+                const int32_t* big = w.pair_lists + w.list_stride;
+                size_t lds_s = (size_t)(64 / w.narrow_group) * w.stage_small * sizeof(V3);
                 switch (w.narrow_group) {
-                case 4: hipLaunchKernelGGL((k_gjk_epa<4, true>), nb, t64, lds, st, w); break;
-                case 8: hipLaunchKernelGGL((k_gjk_epa<8, true>), nb, t64, lds, st, w); break;
-                case 16: hipLaunchKernelGGL((k_gjk_epa<16, true>), nb, t64, lds, st, w); break;
+                case 4: hipLaunchKernelGGL((k_gjk_epa<4, true>), nb, t64, lds_s, st, w, small, w.counters + CNT_SMALL, w.stage_small); break;
+                case 8: hipLaunchKernelGGL((k_gjk_epa<8, true>), nb, t64, lds_s, st, w, small, w.counters + CNT_SMALL, w.stage_small); break;
+                default: hipLaunchKernelGGL((k_gjk_epa<16, true>), nb, t64, lds_s, st, w, small, w.counters + CNT_SMALL, w.stage_small); break;
+                }
+                size_t lds_b = (size_t)(64 / 16) * w.stage_cap * sizeof(V3);
+                hipLaunchKernelGGL((k_gjk_epa<16, true>), nb, t64, lds_b, st, w, big, w.counters + CNT_BIG, w.stage_cap);
+            } else if (fused) {
+                switch (w.narrow_group) {
+                case 4: hipLaunchKernelGGL((k_gjk_epa<4, true>), nb, t64, lds, st, w, nullptr, nullptr, w.stage_cap); break;
+                case 8: hipLaunchKernelGGL((k_gjk_epa<8, true>), nb, t64, lds, st, w, nullptr, nullptr, w.stage_cap); break;
+                case 16: hipLaunchKernelGGL((k_gjk_epa<16, true>), nb, t64, lds, st, w, nullptr, nullptr, w.stage_cap); break;
                 default: hipLaunchKernelGGL(k_gjk_epa_lane<true>, nb, t64, 0, st, w); break;
                 }
             } else {
                 switch (w.narrow_group) {
-                case 4: hipLaunchKernelGGL((k_gjk_epa<4, false>), nb, t64, lds, st, w); break;
-                case 8: hipLaunchKernelGGL((k_gjk_epa<8, false>), nb, t64, lds, st, w); break;
-                case 16: hipLaunchKernelGGL((k_gjk_epa<16, false>), nb, t64, lds, st, w); break;
+                case 4: hipLaunchKernelGGL((k_gjk_epa<4, false>), nb, t64, lds, st, w, nullptr, nullptr, w.stage_cap); break;
+                case 8: hipLaunchKernelGGL((k_gjk_epa<8, false>), nb, t64, lds, st, w, nullptr, nullptr, w.stage_cap); break;
+                case 16: hipLaunchKernelGGL((k_gjk_epa<16, false>), nb, t64, lds, st, w, nullptr, nullptr, w.stage_cap); break;
                 default: hipLaunchKernelGGL(k_gjk_epa_lane<false>, nb, t64, 0, st, w); break;
                 }
             }

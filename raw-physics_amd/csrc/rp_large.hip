@@ -358,12 +358,22 @@ void big_islands(const DevWorld& w, const BigScene& b, double dt, hipStream_t st
 // Global live list and transform flags of the scene (after k_scan_live).
 __global__ void k_big_scatter(DevWorld w, BigScene b) {
     int g0 = w.live_prefix[b.s], n = w.live_count[b.s];
-    for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < n; m += gridDim.x * blockDim.x) {
-        int slot = b.pair_base + w.live_local[b.pair_base + m];
-        w.live_global[g0 + m] = make_int2(b.s, slot);
-        int2 p = w.pairs[slot];
-        w.need[b.b0 + p.x] = 1;  // same value from every writer
-        w.need[b.b0 + p.y] = 1;
+    // whole waves iterate together (the size-class lists take one atomic per wave)
+    int stride = gridDim.x * blockDim.x;
+    for (int m0 = blockIdx.x * blockDim.x; m0 < n; m0 += stride) {
+        int m = m0 + threadIdx.x;
+        bool valid = m < n;
+        int b1 = 0, b2 = 0;
+        if (valid) {
+            int slot = b.pair_base + w.live_local[b.pair_base + m];
+            w.live_global[g0 + m] = make_int2(b.s, slot);
+            int2 p = w.pairs[slot];
+            b1 = b.b0 + p.x;
+            b2 = b.b0 + p.y;
+            w.need[b1] = 1;  // same value from every writer
+            w.need[b2] = 1;
+        }
+        if (w.small_nv > 0) classify_pair(w, valid, g0 + m, b1, b2);
     }
 }
 

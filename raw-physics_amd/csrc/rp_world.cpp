@@ -504,6 +504,15 @@ static bool build_image(const HostWorld& hw, DeviceImage& im) {
     int max_nv = 1;
     for (const HostHull& h : hw.hulls) max_nv = std::max(max_nv, (int)h.v.size());
     d.stage_cap = 2 * max_nv;
+    // size classes: with hulls above SMALL_HULL_NV vertices, the pairs of two
+    // smaller hulls get their own list and LDS slices sized for them (if the
+    // large-hull pairs fit a 16-lane schedule)
+    d.small_nv = 0;
+    if (max_nv > SMALL_HULL_NV && (size_t)(64 / 16) * d.stage_cap * sizeof(V3) <= 64 * 1024) {
+        for (const HostHull& h : hw.hulls)
+            if ((int)h.v.size() <= SMALL_HULL_NV) d.small_nv = std::max(d.small_nv, (int)h.v.size());
+    }
+    d.stage_small = 2 * d.small_nv;
     im.pair_capacity = pcap;
     im.pair_base = pair_base;
     d.narrow_group = 1;
@@ -558,6 +567,8 @@ static bool build_image(const HostWorld& hw, DeviceImage& im) {
     ok &= im.alloc(d.ncontact, (size_t)pcap);
     ok &= im.alloc(d.pair_normal, (size_t)pcap);
     ok &= im.alloc(d.mlist, (size_t)pcap);
+    ok &= im.alloc(d.pair_lists, 2 * (size_t)pcap);
+    d.list_stride = pcap;
     ok &= im.alloc(d.counters, CNT_COUNT * kMaxViews);
     ok &= im.alloc(d.contacts, (size_t)pcap * MAXC);
     ok &= im.upload(d.ext, hw.ext);
@@ -833,7 +844,7 @@ static bool build_views(rp_world* w) {
         v.pair_count += s0; v.live_count += s0; v.live_prefix += s0 + k;
         v.scene_nlevels += s0; v.scene_big += s0; v.scene_err += s0;
         v.item_lstart += s0; v.item_cursor += s0;
-        v.live_global += P0; v.ncontact += P0; v.pair_normal += P0; v.mlist += P0;
+        v.live_global += P0; v.ncontact += P0; v.pair_normal += P0; v.mlist += P0; v.pair_lists += P0;
         v.contacts += P0 * MAXC;
         v.counters += CNT_COUNT * k;
         v.xf += im.body_xf_off[b0];

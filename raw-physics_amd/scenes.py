@@ -517,7 +517,8 @@ def ramps(num_scenes: int, first_seed: int = 0):
 LARGE_SCALES = 64  # scale levels of the large scene (bodies of one level share a hull)
 
 
-def large_scene(n: int, seed: int = 0, large_scene_bodies: int = 0, height: float = 40.0) -> World:
+def large_scene(n: int, seed: int = 0, large_scene_bodies: int = 0, height: float = 40.0,
+                ico_subdiv: int = 1) -> World:
     """The large single-scene workload: `n` dynamic bodies, 60% cubes and 40%
     42-vertex icospheres, scale U(0.5, 1.5) (quantised to LARGE_SCALES
     levels so that bodies share hulls), uniformly random orientation, at rest,
@@ -525,12 +526,13 @@ def large_scene(n: int, seed: int = 0, large_scene_bodies: int = 0, height: floa
     (cube.obj x (L + 10, 1, L + 10) at y = -2), L = 100 sqrt(n / 10^5) so the
     density matches 10^5 bodies in 200 x 40 x 200. Positions are rejection-
     sampled so that no two bounding spheres overlap. Mass = hull volume
-    (density 1); friction 0.5/0.3, restitution 0."""
+    (density 1); friction 0.5/0.3, restitution 0. `ico_subdiv` = 2 gives
+    162-vertex icospheres instead."""
     rng = Rng(0x5EED0000 + seed)
     L = 100.0 * math.sqrt(n / 1.0e5)
     w = World(name=f"large_{n}", large_scene_bodies=large_scene_bodies)
     cube = w.shape(cube_corners())
-    ico = w.shape(icosphere_corners(1))
+    ico = w.shape(icosphere_corners(ico_subdiv))
     w.add_body(cube, (L + 10.0, 1.0, L + 10.0), (0.0, -2.0, 0.0), quaternion_new((0.0, 1.0, 0.0), 0.0),
                fixed=True, material=(0.5, 0.3, 0.0))
     unit_volume = {cube: hull_volume(w.shapes[cube], (1.0, 1.0, 1.0)),

@@ -228,6 +228,37 @@ def test_views_bit_exact(views):
         assert g.error(s) == 0
 
 
+def test_size_classes_change_nothing():
+    """With hulls above 64 vertices, the fused schedule visits small-hull and
+    large-hull pairs in two lists (LDS slices sized per list): the same bits
+    as one list (manifold schedule 1), on uneven scenes, in views, and on a
+    large scene of cubes and 162-vertex icospheres beside small scenes,
+    which also matches the oracle."""
+    big = scenes.large_scene(240, seed=3, large_scene_bodies=64, ico_subdiv=2)
+    for seed in range(3):
+        scenes.drop_scene(big, 1100 + seed)
+    for world, frames, views in ((scenes.batched_uneven(8, first_seed=41, max_bodies=48), 40, 1),
+                                 (scenes.batched_uneven(8, first_seed=41, max_bodies=48), 40, 3),
+                                 (big, 30, 1)):
+        for lanes in (8, 16):
+            a, b = SimWorld("rp", world), SimWorld("rp", world)
+            for w_, m in ((a, 0), (b, 1)):
+                assert w_.lib.rp_world_set_narrow_group(w_.h, lanes) == 0
+                assert w_.lib.rp_world_set_manifold_group(w_.h, m) == 0
+                assert w_.lib.rp_world_set_views(w_.h, views) == 0
+            a.step(frames)
+            b.step(frames)
+            bad = state_bits_equal(a.state(), b.state())
+            assert not bad, f"{world.name}, {lanes} lanes, {views} views: {bad}"
+            for s in range(world.num_scenes):
+                assert a.error(s) == b.error(s)
+    o, g = SimWorld("orc", big), SimWorld("rp", big)
+    o.step(30)
+    g.step(30)
+    bad = state_bits_equal(o.state(), g.state())
+    assert not bad, f"{big.name} against the oracle: {bad}"
+
+
 def _large_beside_small():
     w = scenes.large_scene(300, seed=1, large_scene_bodies=64)
     for seed in range(3):
