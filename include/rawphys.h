@@ -332,6 +332,13 @@ int32_t rp_world_views(const rp_world* w);
 /* Solve schedule: lanes serving one scene in the solve kernel (64, 32 or 16,
  * i.e. 1, 2 or 4 scenes per wave). Identical results. Returns 0 or -1. */
 int32_t rp_world_set_solve_lanes(rp_world* w, int32_t lanes_per_scene);
+
+/* Large scenes (coloured solve): the first `classes` colour classes are
+ * solved one launch each over the whole device, the remaining colours and
+ * the serial class by one 1024-thread block with a barrier between classes
+ * (0..25; default 6). Items of one colour are independent, so results are
+ * identical. Returns 0 or -1. */
+int32_t rp_world_set_colour_launches(rp_world* w, int32_t classes);
 int32_t rp_world_solve_lanes(const rp_world* w);
 
 /* Frame launches as a HIP graph (default on): one frame's kernels are

@@ -65,6 +65,7 @@ def library():
         "rp_world_set_manifold_group": (i32, [_P, i32]),
         "rp_world_manifold_group": (i32, [_P]),
         "rp_world_set_solve_lanes": (i32, [_P, i32]),
+        "rp_world_set_colour_launches": (i32, [_P, i32]),
         "rp_world_solve_lanes": (i32, [_P]),
         "rp_world_set_graphs": (None, [_P, i32]),
         "rp_world_set_views": (i32, [_P, i32]),
@@ -262,6 +263,12 @@ class World:
     def set_graphs(self, on: bool):
         """Replay each frame as a captured HIP graph (default) or launch eagerly."""
         self.lib.rp_world_set_graphs(self.h, 1 if on else 0)
+
+    def set_colour_launches(self, classes: int):
+        """Large scenes: colour classes solved one launch each before the
+        one-block tail (0..25, default 6; identical results)."""
+        if self.lib.rp_world_set_colour_launches(self.h, classes) != 0:
+            raise ValueError(f"colour launches must be 0..25, not {classes}")
 
     def set_solve_lanes(self, lanes_per_scene: int):
         if self.lib.rp_world_set_solve_lanes(self.h, lanes_per_scene) != 0:

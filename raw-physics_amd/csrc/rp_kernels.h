@@ -27,6 +27,7 @@ struct DevWorld {
     int small_nv;                   // > 0: pairs of two hulls of <= small_nv vertices go to their own list
     int stage_small;                // LDS vertices per pair of that list: 2 x small_nv
     int solve_lanes;                // lanes per scene in the solve: 64, 32 or 16
+    int big_launch_classes;         // large scenes: colour classes solved one launch each, the rest in one block
     // scenes
     const int32_t* scene_body_off;  // [ns+1]
     const int32_t* scene_pair_base; // [ns+1]  candidate pair capacity prefix: n(n-1)/2 per scene
@@ -97,6 +98,8 @@ constexpr int BIG_COLOURS = 24;             // parallel colours; items beyond th
 constexpr int BIG_SERIAL = BIG_COLOURS;     // class solved by one lane in item order
 constexpr int BIG_CLASSES = BIG_COLOURS + 1;
 constexpr int BIG_COLOUR_ROUNDS = 32;       // Jones-Plassmann rounds per frame
+constexpr int BIG_LAUNCH_CLASSES = 6;       // default: colours launched one by one before the one-block tail
+constexpr int LARGE_CHUNK = 256;            // bodies per wave in a large body's pair scan
 struct BigScene {
     int s, b0, n;                // scene, first body, bodies
     int pair_base, pair_cap;     // slice of the pair arrays
@@ -104,7 +107,9 @@ struct BigScene {
     int T;                       // hash-table buckets (power of two)
     int n_large;                 // bodies tested against all others (radius above the cell bound)
     double cell;                 // grid cell edge
+    int large_chunks;            // LARGE_CHUNK-body chunks per large body's scan
     int32_t* large;              // [n_large] scene-local indices, ascending
+    int32_t* large_cnt;          // [n_large * large_chunks + 1] chunk pair counts, then offsets
     int32_t* is_large;           // [n]
     int32_t* cell_start;         // [T + 1]
     int32_t* cell_cursor;        // [T]

@@ -3,8 +3,9 @@
 // The per-scene wave kernels (k_broad, k_islands, k_levels, k_solve) serialise
 // a scene on one wave, which a 10^5-body scene cannot afford. A large scene
 // instead runs:
-//   broad phase   uniform hash grid over the small bodies, a wave per large
-//                 body, counting sort and per-body sorted segments: the pair
+//   broad phase   uniform hash grid over the small bodies, waves over chunks
+//                 of the later bodies for each large body, counting sort and
+//                 per-body sorted segments: the pair
 //                 list equals the all-pairs test of broad.cpp:6-29 in set and
 //                 order (i-major, j ascending)
 //   islands       lock-free union-find over pairs and constraints, then the
@@ -13,7 +14,8 @@
 //   solve         items (external constraints, live pairs) coloured once per
 //                 frame (Jones-Plassmann on the item conflict graph: items
 //                 sharing a non-fixed body), then solved colour by colour
-//                 with one lane per item. This replaces the reference's
+//                 with one lane per item (the first colours one launch each,
+//                 the small rest on one block). This replaces the reference's
 //                 sequential Gauss-Seidel order, so results differ from it
 //                 within a stated tolerance; they do not depend on thread
 //                 timing (colours and the serial class are deterministic, and
@@ -206,24 +208,42 @@ __global__ void k_big_pairs_small(DevWorld w, BigScene b) {
     }
 }
 
-// Pairs (l, k > l) of large body l: one wave per large body scans every
-// later body in order, so its segment comes out sorted.
+// Pairs (l, k > l) of large body l: the later bodies in chunks of
+// LARGE_CHUNK, one wave per (large body, chunk) scanning its chunk in order.
+// PASS 0 counts each chunk; the counts are scanned (per large body: the
+// difference from its first chunk) and PASS 1 writes each chunk's hits at
+// its offset, so the segment comes out sorted.
 template <int PASS>
 __global__ void __launch_bounds__(64) k_big_pairs_large(DevWorld w, BigScene b) {
-    int l = b.large[blockIdx.x];
+    int li = blockIdx.x / b.large_chunks, c = blockIdx.x % b.large_chunks;
+    int l = b.large[li];
     const Body& e1 = w.body[b.b0 + l];
     int cnt = 0;
-    int off = (PASS == 1) ? b.pair_off[l] : 0;
+    int off = 0;
+    if (PASS == 1) {
+        const int32_t* row = b.large_cnt + (size_t)li * b.large_chunks;
+        off = b.pair_off[l] + row[c] - row[0];
+    }
     int2* out = w.pairs + b.pair_base;
-    for (int k0 = l + 1; k0 < b.n; k0 += 64) {
-        int k = k0 + (int)threadIdx.x;
-        bool hit = k < b.n && broad_hit(e1, w.body[b.b0 + k]);
+    int k0 = l + 1 + c * LARGE_CHUNK;
+    int k1 = min(b.n, k0 + LARGE_CHUNK);
+    for (int kb = k0; kb < k1; kb += 64) {
+        int k = kb + (int)threadIdx.x;
+        bool hit = k < k1 && broad_hit(e1, w.body[b.b0 + k]);
         int total;
         int slot = wave_compact(hit, &total);
         if (PASS == 1 && hit && off + cnt + slot < b.pair_cap) out[off + cnt + slot] = make_int2(l, k);
         cnt += total;
     }
-    if (PASS == 0 && threadIdx.x == 0) b.pair_off[l] = cnt;
+    if (PASS == 0 && threadIdx.x == 0) b.large_cnt[blockIdx.x] = cnt;
+}
+
+// Segment lengths of the large bodies from their scanned chunk counts.
+__global__ void k_big_large_total(DevWorld w, BigScene b) {
+    for (int li = blockIdx.x * blockDim.x + threadIdx.x; li < b.n_large; li += gridDim.x * blockDim.x) {
+        const int32_t* row = b.large_cnt + (size_t)li * b.large_chunks;
+        b.pair_off[b.large[li]] = row[b.large_chunks] - row[0];
+    }
 }
 
 __global__ void k_big_pair_count(DevWorld w, BigScene b) {
@@ -240,10 +260,15 @@ void big_broad(const DevWorld& w, const BigScene& b, hipStream_t st) {
     hipLaunchKernelGGL(k_copy, dim3(grid_for(b.T, 256)), dim3(256), 0, st, b.cell_start, b.cell_cursor, b.T);
     hipLaunchKernelGGL(k_big_fill, dim3(gb), dim3(256), 0, st, w, b);
     hipLaunchKernelGGL(k_big_pairs_small<0>, dim3(gb), dim3(256), 0, st, w, b);
-    if (b.n_large > 0) hipLaunchKernelGGL(k_big_pairs_large<0>, dim3(b.n_large), dim3(64), 0, st, w, b);
+    int nlc = b.n_large * b.large_chunks;
+    if (b.n_large > 0) {
+        hipLaunchKernelGGL(k_big_pairs_large<0>, dim3(nlc), dim3(64), 0, st, w, b);
+        scan(b.large_cnt, b.large_cnt, nlc, b.scan_blocks, st);
+        hipLaunchKernelGGL(k_big_large_total, dim3(grid_for(b.n_large, 256)), dim3(256), 0, st, w, b);
+    }
     scan(b.pair_off, b.pair_off, b.n, b.scan_blocks, st);
     hipLaunchKernelGGL(k_big_pairs_small<1>, dim3(gb), dim3(256), 0, st, w, b);
-    if (b.n_large > 0) hipLaunchKernelGGL(k_big_pairs_large<1>, dim3(b.n_large), dim3(64), 0, st, w, b);
+    if (b.n_large > 0) hipLaunchKernelGGL(k_big_pairs_large<1>, dim3(nlc), dim3(64), 0, st, w, b);
     hipLaunchKernelGGL(k_big_pair_count, dim3(1), dim3(1), 0, st, w, b);
 }
 
@@ -525,10 +550,26 @@ __global__ void k_big_order(DevWorld w, BigScene b, int collisions) {
     const int* ls = w.item_lstart + b.item_base + b.s;
     int* cur = w.item_cursor + b.item_base + b.s;
     int* order = w.item_order + b.item_base;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < K; t += gridDim.x * blockDim.x) {
-        int c = colour[t];
-        int pos = (c == BIG_SERIAL) ? ls[BIG_SERIAL] + b.tmp[t] : atomicAdd(&cur[c], 1);
-        order[pos] = item_code(w, b, t);
+    int lane = lane_id();
+    unsigned long long below = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
+    // whole waves iterate (the cursor atomics are aggregated per wave and colour)
+    for (int t0 = blockIdx.x * blockDim.x; t0 < K; t0 += gridDim.x * blockDim.x) {
+        int t = t0 + threadIdx.x;
+        bool valid = t < K;
+        int c = valid ? colour[t] : BIG_SERIAL;
+        int pos = valid ? ls[BIG_SERIAL] + b.tmp[t] : 0;
+        unsigned long long todo = __ballot(c != BIG_SERIAL);
+        while (todo) {  // one round per colour present in the wave
+            int leader = __ffsll((long long)todo) - 1;
+            int cl = __shfl(c, leader);
+            unsigned long long m = __ballot(c == cl);
+            int base = 0;
+            if (lane == leader) base = atomicAdd(&cur[cl], __popcll(m));
+            base = __shfl(base, leader);
+            if (c == cl) pos = base + __popcll(m & below);
+            todo &= ~m;
+        }
+        if (valid) order[pos] = item_code(w, b, t);
     }
 }
 
@@ -589,6 +630,31 @@ __global__ void __launch_bounds__(64) k_big_class(DevWorld w, BigScene b, double
     if (err) atomicOr(&w.scene_err[b.s], err);
 }
 
+// Colour classes c0.. and then the serial class, on one block: a block
+// barrier between classes instead of a launch per class.
+template <bool kExt, bool kVel>
+__global__ void __launch_bounds__(1024) k_big_tail(DevWorld w, BigScene b, double h, int c0) {
+    const int* ls = w.item_lstart + b.item_base + b.s;
+    const int* order = w.item_order + b.item_base;
+    uint32_t err = 0;
+    for (int c = c0; c < BIG_COLOURS; ++c) {
+        int i0 = ls[c], i1 = ls[c + 1];
+        if (i0 == i1) continue;  // the same for every thread
+        for (int idx = i0 + (int)threadIdx.x; idx < i1; idx += blockDim.x) {
+            if (kVel) solve_item_vel(w, order[idx], b.b0, h);
+            else solve_item_pos<kExt>(w, order[idx], b.b0, h, err);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        for (int idx = ls[BIG_SERIAL]; idx < ls[BIG_SERIAL + 1]; ++idx) {
+            if (kVel) solve_item_vel(w, order[idx], b.b0, h);
+            else solve_item_pos<kExt>(w, order[idx], b.b0, h, err);
+        }
+    }
+    if (err) atomicOr(&w.scene_err[b.s], err);
+}
+
 __global__ void k_big_velocity_update(DevWorld w, BigScene b, double h) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < b.n; i += gridDim.x * blockDim.x)
         velocity_update_body(w.body[b.b0 + i], h);
@@ -598,14 +664,22 @@ void big_solve(const DevWorld& w, const BigScene& b, const FrameParams& fp, hipS
     int gi = std::min(grid_for(b.item_cap, 64), 2048);
     bool ext = w.n_ext > 0;
     if (ext) hipLaunchKernelGGL(k_big_ext_zero, dim3(grid_for(w.n_ext, 256)), dim3(256), 0, st, w, b);
-    for (unsigned it = 0; it < fp.iters; ++it)
-        for (int c = 0; c < BIG_CLASSES; ++c) {
+    // classes 0..L-1 one launch each (the large ones), the rest on one block
+    int L = std::min(std::max(w.big_launch_classes, 0), BIG_CLASSES);
+    for (unsigned it = 0; it < fp.iters; ++it) {
+        for (int c = 0; c < L; ++c) {
             if (ext) hipLaunchKernelGGL((k_big_class<true, false>), dim3(gi), dim3(64), 0, st, w, b, fp.h, c);
             else hipLaunchKernelGGL((k_big_class<false, false>), dim3(gi), dim3(64), 0, st, w, b, fp.h, c);
         }
+        if (L < BIG_CLASSES) {
+            if (ext) hipLaunchKernelGGL((k_big_tail<true, false>), dim3(1), dim3(1024), 0, st, w, b, fp.h, L);
+            else hipLaunchKernelGGL((k_big_tail<false, false>), dim3(1), dim3(1024), 0, st, w, b, fp.h, L);
+        }
+    }
     hipLaunchKernelGGL(k_big_velocity_update, dim3(grid_for(b.n, 256)), dim3(256), 0, st, w, b, fp.h);
-    for (int c = 0; c < BIG_CLASSES; ++c)
+    for (int c = 0; c < L; ++c)
         hipLaunchKernelGGL((k_big_class<false, true>), dim3(gi), dim3(64), 0, st, w, b, fp.h, c);
+    if (L < BIG_CLASSES) hipLaunchKernelGGL((k_big_tail<false, true>), dim3(1), dim3(1024), 0, st, w, b, fp.h, L);
 }
 
 }  // namespace rp

@@ -525,6 +525,7 @@ static bool build_image(const HostWorld& hw, DeviceImage& im) {
     d.manifold_blocks = narrow_blocks_for(d, 1, pcap);
     if (!im.ensure_group_work()) return false;
     d.solve_lanes = 64;
+    d.big_launch_classes = BIG_LAUNCH_CLASSES;
 
     bool ok = true;
     ok &= im.upload(d.scene_body_off, hw.scene_body_off);
@@ -599,6 +600,13 @@ static bool build_image(const HostWorld& hw, DeviceImage& im) {
             if (r[i] > small_bound) { large.push_back(i); is_large[i] = 1; }
         const int32_t* cp;
         ok &= im.upload(cp, large); bs.large = const_cast<int32_t*>(cp);
+        bs.large_chunks = (bs.n + LARGE_CHUNK - 1) / LARGE_CHUNK;
+        int64_t nlc = (int64_t)bs.n_large * bs.large_chunks;
+        if (nlc >= INT32_MAX) {
+            set_msg("too many large bodies in one scene");
+            return false;
+        }
+        ok &= im.alloc(bs.large_cnt, (size_t)nlc + 1);
         ok &= im.upload(cp, is_large); bs.is_large = const_cast<int32_t*>(cp);
         int64_t tmp_n = std::max<int64_t>(bs.pair_cap, bs.item_cap) + 1;
         ok &= im.alloc(bs.cell_start, (size_t)bs.T + 1);
@@ -610,7 +618,7 @@ static bool build_image(const HostWorld& hw, DeviceImage& im) {
         ok &= im.alloc(bs.bi_cursor, (size_t)bs.n);
         ok &= im.alloc(bs.bi_items, 2 * (size_t)bs.item_cap);
         ok &= im.alloc(bs.remaining, 1);
-        int64_t scan_n = std::max<int64_t>({tmp_n, (int64_t)bs.T, (int64_t)bs.n + 1});
+        int64_t scan_n = std::max<int64_t>({tmp_n, (int64_t)bs.T, (int64_t)bs.n + 1, nlc + 1});
         ok &= im.alloc(bs.scan_blocks, (size_t)scan_block_count(scan_n));
     }
     im.ok = ok;
@@ -1401,6 +1409,14 @@ int32_t rp_world_set_solve_lanes(rp_world* w, int32_t lanes_per_scene) {
 }
 
 int32_t rp_world_solve_lanes(const rp_world* w) { return w->im.d.solve_lanes; }
+
+int32_t rp_world_set_colour_launches(rp_world* w, int32_t classes) {
+    if (classes < 0 || classes > BIG_CLASSES) return -1;
+    (void)hipStreamSynchronize(w->stream);
+    w->im.d.big_launch_classes = classes;
+    ++w->sched_gen;
+    return 0;
+}
 
 int32_t rp_world_set_views(rp_world* w, int32_t views) {
     if (views < 0 || views > kMaxViews) return -1;

@@ -55,6 +55,24 @@ def test_large_scene_is_deterministic():
     assert a.error(0) == 0
 
 
+def test_colour_launch_schedules_give_identical_bits():
+    """Colour classes one launch each (25), the default split (6 launches,
+    then one block for the small classes and the serial class), and every
+    class on one block (0): items of a colour are independent, so all three
+    give the same bits."""
+    w = _large(True)
+    runs = []
+    for classes in (25, 6, 0):
+        g = SimWorld("rp", w)
+        assert g.lib.rp_world_set_colour_launches(g.h, classes) == 0
+        g.step(45)
+        runs.append(g.state())
+        assert g.error(0) == 0
+    assert not state_bits_equal(runs[0], runs[1])
+    assert not state_bits_equal(runs[0], runs[2])
+    assert g.lib.rp_world_set_colour_launches(g.h, 26) != 0
+
+
 def test_large_scene_one_frame_within_tolerance():
     w = _large(True)
     g = SimWorld("rp", w)
