@@ -121,13 +121,14 @@ template <int CTRL> __device__ __forceinline__ double dpp_double(double v) {
 }
 // (dot, index) of the partner lane, kept if its dot is larger or equal with a
 // lower index; an index < 0 means "none yet".
+// Written as selects over a non-short-circuit predicate, so that it compiles
+// to compares and v_cndmask instead of a branch per term.
 template <int CTRL> __device__ __forceinline__ void argmax_step(double& m, int& sel) {
     double om = dpp_double<CTRL>(m);
     int os = dpp_int<CTRL>(sel);
-    if (os >= 0 && (sel < 0 || om > m || (om == m && os < sel))) {
-        m = om;
-        sel = os;
-    }
+    bool take = (os >= 0) & ((sel < 0) | (om > m) | ((om == m) & (os < sel)));
+    m = take ? om : m;
+    sel = take ? os : sel;
 }
 // First maximum over the G lanes of the group (G = 4, 8 or 16): after each
 // step every lane holds the best of a block of 2, 4, 8, 16 lanes.
@@ -163,7 +164,9 @@ RP_HD int support_index(const HullView& h, V3 d, uint32_t& err) {
         int gl = (int)(__lane_id() & (G - 1));
         for (int i = gl; i < h.nv; i += G) {
             double dd = dot(h.tv[i], d);
-            if (dd > max_dot) { sel = i; max_dot = dd; }
+            bool t = dd > max_dot;
+            sel = t ? i : sel;
+            max_dot = t ? dd : max_dot;
         }
         group_argmax<G>(max_dot, sel);
     }
@@ -202,27 +205,29 @@ RP_HD void support_index2(const HullView& a, V3 d1, const HullView& b, V3 d2, in
         }
     } else {
         // a lane visits its vertices gl, gl + G, ... in order, four of each
-        // hull per batch, all loaded before any is compared
+        // hull per batch, all loaded before any is compared; positions past a
+        // hull's end load its last vertex (in bounds) and are masked out of
+        // the comparison, so the batch has no branches
         int gl = (int)(__lane_id() & (G - 1));
         for (int i = gl; i < n; i += 4 * G) {
             V3 va[4], vb[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 int j = i + k * G;
-                if (j < a.nv) va[k] = a.tv[j];
-                if (j < b.nv) vb[k] = b.tv[j];
+                va[k] = a.tv[j < a.nv ? j : a.nv - 1];
+                vb[k] = b.tv[j < b.nv ? j : b.nv - 1];
             }
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 int j = i + k * G;
-                if (j < a.nv) {
-                    double dd = dot(va[k], d1);
-                    if (dd > m1) { s1 = j; m1 = dd; }
-                }
-                if (j < b.nv) {
-                    double dd = dot(vb[k], d2);
-                    if (dd > m2) { s2 = j; m2 = dd; }
-                }
+                double da = dot(va[k], d1);
+                bool ta = (j < a.nv) & (da > m1);
+                s1 = ta ? j : s1;
+                m1 = ta ? da : m1;
+                double db = dot(vb[k], d2);
+                bool tb = (j < b.nv) & (db > m2);
+                s2 = tb ? j : s2;
+                m2 = tb ? db : m2;
             }
         }
         group_argmax<G>(m1, s1);
@@ -529,9 +534,16 @@ struct EpaPoly {
             dist[nf] = fd;
             ++nf;
         }
+        // first minimum: selects over the distances, then one normal load
         min_d = DBL_MAX;
-        for (int i = 0; i < nf; ++i)
-            if (dist[i] < min_d) { min_d = dist[i]; min_n = normals[i]; }
+        int mi = -1;
+        for (int i = 0; i < nf; ++i) {
+            double di = dist[i];
+            bool t = di < min_d;
+            min_d = t ? di : min_d;
+            mi = t ? i : mi;
+        }
+        if (mi >= 0) min_n = normals[mi];
         ne = 0;
         return 2;
     }
@@ -720,7 +732,9 @@ __device__ __forceinline__ int best_face_group(int sidx, const HullView& h, V3 n
     int pos = -1;
     for (int k = gl; k < cnt; k += G) {
         double proj = dot(face_normal_world(h, h.vf[b + k]), n);
-        if (proj > max_proj) { max_proj = proj; pos = k; }
+        bool t = proj > max_proj;
+        max_proj = t ? proj : max_proj;
+        pos = t ? k : pos;
     }
     group_argmax<G>(max_proj, pos);
     if (pos < 0) { err |= ERR_SELECT; return 0; }
@@ -762,9 +776,13 @@ RP_HD int manifold(const HullView& h1, const HullView& h2, V3 normal, ContactPt*
             V3 cn = nrm(cross(ed1, ed2));
             V3 cni = inv3(cn);
             double dd = dot(cn, normal);
-            if (dd > max_dot) { max_dot = dd; best = 2 * k; }
+            bool t = dd > max_dot;
+            max_dot = t ? dd : max_dot;
+            best = t ? 2 * k : best;
             dd = dot(cni, normal);
-            if (dd > max_dot) { max_dot = dd; best = 2 * k + 1; }
+            t = dd > max_dot;
+            max_dot = t ? dd : max_dot;
+            best = t ? 2 * k + 1 : best;
         }
         group_argmax<G>(max_dot, best);
         if (best < 0) {
