@@ -385,7 +385,10 @@ __device__ __forceinline__ void gjk_epa_pairs(DevWorld& w, const int32_t* list, 
             ContactPt local[G == 1 ? MAXC : 1];
             ContactPt* cts = G == 1 ? local : (ContactPt*)(work + CLIP_BYTES);
             int nc = 0;
-            if (r == 1) nc = manifold<G>(h1, h2, n, cts, err, work);
+            if (r == 1) {
+                int used = h1.nv + h2.nv;  // the rest of the pair's LDS slice is free
+                nc = manifold<G>(h1, h2, n, cts, err, work, st + used, G > 1 ? stage - used : 0);
+            }
             if (gl == 0) {
                 const Body& e1 = w.body[b1];
                 const Body& e2 = w.body[b2];
@@ -439,7 +442,8 @@ __global__ void __launch_bounds__(64) k_manifold(DevWorld w) {
         char* work = G > 1 ? w.group_work + (size_t)(blockIdx.x * NG + grp) * GROUP_WORK_BYTES : nullptr;
         ContactPt local[G == 1 ? MAXC : 1];
         ContactPt* cts = G == 1 ? local : (ContactPt*)(work + CLIP_BYTES);
-        int nc = manifold<G>(h1, h2, w.pair_normal[g], cts, err, work);
+        int used = h1.nv + h2.nv;
+        int nc = manifold<G>(h1, h2, w.pair_normal[g], cts, err, work, st + used, G > 1 ? w.stage_cap - used : 0);
         if (gl == 0) {
             const Body& e1 = w.body[b1];
             const Body& e2 = w.body[b2];

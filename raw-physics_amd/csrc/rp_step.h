@@ -684,6 +684,91 @@ RP_HD int clip_pass(const V3* in, int n, const Plane& p, V3* out, bool remove, u
     return m;
 }
 
+// Lane-group clipping (G > 1): the polygons live in LDS, and each pass gives
+// every lane one edge (start = in[j - 1], end = in[j]); a lane's 0-2 output
+// points go to their place in the sequential output order by a prefix sum
+// over the group, so the polygon equals clip_pass's point for point.
+__device__ __forceinline__ void group_mem_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+template <int G> __device__ __forceinline__ int group_inclusive_sum(int v) {
+    int gl = (int)(__lane_id() & (G - 1));
+#pragma unroll
+    for (int d = 1; d < G; d <<= 1) {
+        int o = __shfl_up(v, d, G);
+        v += gl >= d ? o : 0;
+    }
+    return v;
+}
+// One pass (clip_pass's rules) over `in` (n points) into `out` (cap points).
+// Returns the output count; more than cap sets `ovf` (no point is lost:
+// the caller redoes the clipping in CLIP_MAX buffers).
+template <int G>
+__device__ int clip_pass_group(const V3* in, int n, const Plane& p, V3* out, bool remove, int cap, bool& ovf) {
+    int gl = (int)(__lane_id() & (G - 1));
+    int m = 0;
+    for (int c0 = 0; c0 < n; c0 += G) {
+        int j = c0 + gl;
+        int cnt = 0;
+        V3 o0 = v3(0.0, 0.0, 0.0), o1 = o0;
+        if (j < n) {
+            V3 start = in[j == 0 ? n - 1 : j - 1], end = in[j];
+            bool s_in = point_in_plane(p, start);
+            bool e_in = point_in_plane(p, end);
+            V3 t;
+            if (remove) {
+                if (e_in) { o0 = end; cnt = 1; }
+            } else if (s_in && e_in) {
+                o0 = end; cnt = 1;
+            } else if (s_in && !e_in) {
+                if (plane_edge(p, start, end, t)) { o0 = t; cnt = 1; }
+            } else if (!s_in && e_in) {
+                if (plane_edge(p, start, end, t)) { o0 = t; o1 = end; cnt = 2; }
+                else { o0 = end; cnt = 1; }
+            }
+        }
+        int incl = group_inclusive_sum<G>(cnt);
+        int total = __shfl(incl, G - 1, G);
+        int pos = m + incl - cnt;
+        if (cnt >= 1 && pos < cap) out[pos] = o0;
+        if (cnt == 2 && pos + 1 < cap) out[pos + 1] = o1;
+        m += total;
+    }
+    if (m > cap) ovf = true;
+    group_mem_sync();
+    return m;
+}
+// clip_incident on the group's lanes with LDS buffers of `cap` points;
+// -1 when they overflow. Every lane of the group calls it.
+template <int G>
+__device__ int clip_incident_group(const HullView& rh, int rf, const HullView& ih, int inf, const Plane& rp, V3* bufA,
+                                   V3* bufB, int cap, const V3*& res) {
+    int gl = (int)(__lane_id() & (G - 1));
+    int f0 = ih.fe_off[inf], n = ih.fe_off[inf + 1] - f0;
+    if (n > cap) return -1;
+    for (int i = gl; i < n; i += G) bufA[i] = ih.tv[ih.fe[f0 + i]];
+    group_mem_sync();
+    bool ovf = false;
+    V3* in = bufA;
+    V3* outp = bufB;
+    for (int k = rh.ff_off[rf]; k < rh.ff_off[rf + 1]; ++k) {
+        if (n == 0) break;
+        int nf = rh.ff[k];
+        Plane p;
+        p.point = rh.tv[rh.fe[rh.fe_off[nf]]];
+        p.normal = inv3(face_normal_world(rh, nf));
+        n = clip_pass_group<G>(in, n, p, outp, false, cap, ovf);
+        if (ovf) return -1;
+        V3* t = in; in = outp; outp = t;
+    }
+    n = clip_pass_group<G>(in, n, rp, outp, true, cap, ovf);
+    if (ovf) return -1;
+    res = outp;
+    return n;
+}
+
 // get_face_with_most_fitting_normal (clipping.cpp:136-152)
 RP_HD int best_face(int sidx, const HullView& h, V3 n, uint32_t& err) {
     double max_proj = -DBL_MAX;
@@ -741,9 +826,68 @@ __device__ __forceinline__ int best_face_group(int sidx, const HullView& h, V3 n
     return h.vf[b + pos];
 }
 
+// The incident face clipped against the reference face's boundary planes
+// (clipping.cpp:121-134, 294) and the points above the reference plane
+// removed (clipping.cpp:296-302), by one lane, in two caller-owned buffers
+// of CLIP_MAX points; `res` is set to the one holding the result.
+RP_HD int clip_serial(const HullView& rh, int rf, const HullView& ih, int inf, const Plane& rp, V3* bufA, V3* bufB,
+                      const V3*& res, uint32_t& err) {
+    int n = 0;
+    for (int i = ih.fe_off[inf]; i < ih.fe_off[inf + 1]; ++i) {
+        if (n < CLIP_MAX) bufA[n++] = ih.tv[ih.fe[i]]; else err |= ERR_CLIP_CAPACITY;
+    }
+    // clip against the reference face's boundary planes (clipping.cpp:121-134, 294)
+    V3* in = bufA;
+    V3* outp = bufB;
+    for (int k = rh.ff_off[rf]; k < rh.ff_off[rf + 1]; ++k) {
+        if (n == 0) break;
+        int nf = rh.ff[k];
+        Plane p;
+        p.point = rh.tv[rh.fe[rh.fe_off[nf]]];
+        p.normal = inv3(face_normal_world(rh, nf));
+        n = clip_pass(in, n, p, outp, false, err);
+        V3* t = in; in = outp; outp = t;
+    }
+    n = clip_pass(in, n, rp, outp, true, err);
+    res = outp;
+    return n;
+}
+
+// get_closest_point_polygon (clipping.cpp:115-119) and the contact points
+// (clipping.cpp:304-329) of the clipped polygon.
+RP_HD int contacts_from_clip(const V3* pts, int n, const Plane& rp, V3 normal, bool ref1, ContactPt* out,
+                             uint32_t& err) {
+    int nc = 0;
+    double pd = dot(scl(-1.0, rp.normal), rp.point);
+    for (int i = 0; i < n; ++i) {
+        V3 pt = pts[i];
+        V3 cp = sub(pt, scl(dot(rp.normal, pt) + pd, rp.normal));
+        V3 diff = sub(pt, cp);
+        double pen;
+        ContactPt c;
+        if (ref1) {
+            pen = dot(diff, normal);
+            c.p1 = sub(pt, scl(pen, normal));
+            c.p2 = pt;
+        } else {
+            pen = -dot(diff, normal);
+            c.p1 = pt;
+            c.p2 = add(pt, scl(pen, normal));
+        }
+        c.n = normal;
+        if (pen < 0.0) {
+            if (nc < MAXC) out[nc++] = c; else err |= ERR_CONTACT_CAPACITY;
+        }
+    }
+    return nc;
+}
+
+// G > 1: the group clips in `spare` (spare_n points: the unused end of the
+// pair's LDS slice) when both polygons fit there, else lane 0 clips in
+// `work`; the contact points are lane 0's.
 template <int G = 1>
 RP_HD int manifold(const HullView& h1, const HullView& h2, V3 normal, ContactPt* out, uint32_t& err,
-                   char* work = nullptr) {
+                   char* work = nullptr, V3* spare = nullptr, int spare_n = 0) {
     const double EPSILON = 0.0001;
     V3 inormal = inv3(normal);
     int s1, s2;
@@ -793,8 +937,6 @@ RP_HD int manifold(const HullView& h1, const HullView& h2, V3 normal, ContactPt*
             V3 cn = nrm(cross(sub(sp1, h1.tv[e1b]), sub(sp2, h2.tv[e2b])));
             edge_n = (best & 1) ? inv3(cn) : cn;
         }
-        // the rest (clipping, contact points) runs on lane 0 of the group
-        if (gl != 0) return 0;
     } else {
         V3 sp1 = h1.tv[s1], sp2 = h2.tv[s2];
         double max_dot = -DBL_MAX;
@@ -816,13 +958,15 @@ RP_HD int manifold(const HullView& h1, const HullView& h2, V3 normal, ContactPt*
     double d1 = dot(face_normal_world(h1, f1), normal);
     double d2 = dot(face_normal_world(h2, f2), inormal);
     double de = dot(edge_n, normal);
-    int nc = 0;
     if (de > d1 + EPSILON && de > d2 + EPSILON) {
         V3 l1, l2;
         V3 p1 = h1.tv[e1a];
         V3 dd1 = sub(h1.tv[e1b], p1);
         V3 p2 = h2.tv[e2a];
         V3 dd2 = sub(h2.tv[e2b], p2);
+        if constexpr (G > 1) {
+            if ((__lane_id() & (G - 1)) != 0) return 0;  // lane 0 writes the contact
+        }
         if (!skew_lines(p1, dd1, p2, dd2, l1, l2)) { err |= ERR_SKEW_LINES; return 0; }
         out[0].p1 = l1; out[0].p2 = l2; out[0].n = normal;
         return 1;
@@ -831,54 +975,26 @@ RP_HD int manifold(const HullView& h1, const HullView& h2, V3 normal, ContactPt*
     const HullView& rh = ref1 ? h1 : h2;
     const HullView& ih = ref1 ? h2 : h1;
     int rf = ref1 ? f1 : f2, inf = ref1 ? f2 : f1;
-    // clipping polygons: private arrays, or (G > 1) the group's working memory
-    V3 localA[G == 1 ? CLIP_MAX : 1], localB[G == 1 ? CLIP_MAX : 1];
-    V3* bufA = G == 1 ? localA : (V3*)work;
-    V3* bufB = G == 1 ? localB : (V3*)work + CLIP_MAX;
-    int n = 0;
-    for (int i = ih.fe_off[inf]; i < ih.fe_off[inf + 1]; ++i) {
-        if (n < CLIP_MAX) bufA[n++] = ih.tv[ih.fe[i]]; else err |= ERR_CLIP_CAPACITY;
-    }
-    // clip against the reference face's boundary planes (clipping.cpp:121-134, 294)
-    V3* in = bufA;
-    V3* outp = bufB;
-    for (int k = rh.ff_off[rf]; k < rh.ff_off[rf + 1]; ++k) {
-        if (n == 0) break;
-        int nf = rh.ff[k];
-        Plane p;
-        p.point = rh.tv[rh.fe[rh.fe_off[nf]]];
-        p.normal = inv3(face_normal_world(rh, nf));
-        n = clip_pass(in, n, p, outp, false, err);
-        V3* t = in; in = outp; outp = t;
-    }
     // remove the points above the reference plane (clipping.cpp:296-302)
     Plane rp;
     rp.normal = inv3(face_normal_world(rh, rf));
     rp.point = rh.tv[rh.fe[rh.fe_off[rf]]];
-    n = clip_pass(in, n, rp, outp, true, err);
-    // get_closest_point_polygon (clipping.cpp:115-119) and contact points (clipping.cpp:304-329)
-    double pd = dot(scl(-1.0, rp.normal), rp.point);
-    for (int i = 0; i < n; ++i) {
-        V3 pt = outp[i];
-        V3 cp = sub(pt, scl(dot(rp.normal, pt) + pd, rp.normal));
-        V3 diff = sub(pt, cp);
-        double pen;
-        ContactPt c;
-        if (ref1) {
-            pen = dot(diff, normal);
-            c.p1 = sub(pt, scl(pen, normal));
-            c.p2 = pt;
-        } else {
-            pen = -dot(diff, normal);
-            c.p1 = pt;
-            c.p2 = add(pt, scl(pen, normal));
-        }
-        c.n = normal;
-        if (pen < 0.0) {
-            if (nc < MAXC) out[nc++] = c; else err |= ERR_CONTACT_CAPACITY;
-        }
+    const V3* outp = nullptr;
+    int n = -1;
+    if constexpr (G > 1) {
+        int cap = spare_n / 2 < CLIP_MAX ? spare_n / 2 : CLIP_MAX;
+        if (cap >= 8) n = clip_incident_group<G>(rh, rf, ih, inf, rp, spare, spare + cap, cap, outp);
+        if ((__lane_id() & (G - 1)) != 0) return 0;  // lane 0 goes on
     }
-    return nc;
+    // one lane: private arrays (G = 1) or the group's working memory; the
+    // arrays live here, in the frame that reads the result
+    V3 localA[G == 1 ? CLIP_MAX : 1], localB[G == 1 ? CLIP_MAX : 1];
+    if (n < 0) {
+        V3* bufA = G == 1 ? localA : (V3*)work;
+        V3* bufB = G == 1 ? localB : (V3*)work + CLIP_MAX;
+        n = clip_serial(rh, rf, ih, inf, rp, bufA, bufB, outp, err);
+    }
+    return contacts_from_clip(outp, n, rp, normal, ref1, out, err);
 }
 
 // collider_get_contacts for two hulls (collider.cpp:523-558): GJK, EPA, manifold.
