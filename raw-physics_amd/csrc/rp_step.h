@@ -140,6 +140,24 @@ template <int G> __device__ __forceinline__ void group_argmax(double& m, int& se
     if constexpr (G >= 16) argmax_step<DPP_MIRROR>(m, sel);
 }
 
+// (value, index) of the partner lane, kept if its value is smaller, or equal
+// with a lower index; an index < 0 means "none yet".
+template <int CTRL> __device__ __forceinline__ void argmin_step(double& m, int& sel) {
+    double om = dpp_double<CTRL>(m);
+    int os = dpp_int<CTRL>(sel);
+    bool take = (os >= 0) & ((sel < 0) | (om < m) | ((om == m) & (os < sel)));
+    m = take ? om : m;
+    sel = take ? os : sel;
+}
+// First minimum over the G lanes of the group.
+template <int G> __device__ __forceinline__ void group_argmin(double& m, int& sel) {
+    static_assert(G == 4 || G == 8 || G == 16, "lane groups of 4, 8 or 16");
+    argmin_step<DPP_XOR1>(m, sel);
+    argmin_step<DPP_XOR2>(m, sel);
+    if constexpr (G >= 8) argmin_step<DPP_HALF_MIRROR>(m, sel);
+    if constexpr (G >= 16) argmin_step<DPP_MIRROR>(m, sel);
+}
+
 template <int G = 1>
 RP_HD int support_index(const HullView& h, V3 d, uint32_t& err) {
     int sel = -1;
@@ -475,6 +493,33 @@ struct EpaPoly {
         }
         return true;
     }
+    // add_edge (epa.cpp:79-110): an edge equal to a listed one, by index or
+    // by position, cancels it (swap-remove), else it is appended; false when
+    // the list is full.
+    RP_HD bool add_edge(int2 e, uint32_t& err) {
+        for (int j = 0; j < ne; ++j) {
+            int2 cur = edges[j];
+            bool hit = (e.x == cur.x && e.y == cur.y) || (e.x == cur.y && e.y == cur.x);
+            if (!hit) {
+                V3 c1 = poly[cur.x], c2 = poly[cur.y], e1 = poly[e.x], e2 = poly[e.y];
+                hit = (veq(c1, e1) && veq(c2, e2)) || (veq(c1, e2) && veq(c2, e1));
+            }
+            if (hit) {
+                --ne;
+                edges[j] = edges[ne];
+                return true;
+            }
+        }
+        if (ne == EPA_MAXE) { err |= ERR_EPA_CAPACITY; return false; }
+        edges[ne++] = e;
+        return true;
+    }
+    // triangle_centroid (epa.cpp:112-116) test: does face i see sp?
+    RP_HD bool sees(int i, V3 sp) const {
+        I3 f = faces[i];
+        V3 c = scl(1.0 / 3.0, add(add(poly[f.y], poly[f.z]), poly[f.x]));
+        return dot(normals[i], sub(sp, c)) > 0.0;
+    }
     // One iteration with the support point sp in direction min_n:
     // 1 = converged (n_out, pen_out set), 2 = expanded, -1 = error.
     RP_HD int step(V3 sp, V3& n_out, double& pen_out, uint32_t& err) {
@@ -492,30 +537,9 @@ struct EpaPoly {
             // triangle_centroid (epa.cpp:112-116)
             V3 c = scl(1.0 / 3.0, add(add(poly[f.y], poly[f.z]), poly[f.x]));
             if (dot(n, sub(sp, c)) > 0.0) {
-                int2 es[3] = {make_int2(f.x, f.y), make_int2(f.y, f.z), make_int2(f.z, f.x)};
-                for (int k = 0; k < 3; ++k) {
-                    // add_edge (epa.cpp:79-110): cancel by index or by position, swap-remove
-                    int2 e = es[k];
-                    bool removed = false;
-                    for (int j = 0; j < ne; ++j) {
-                        int2 cur = edges[j];
-                        bool hit = (e.x == cur.x && e.y == cur.y) || (e.x == cur.y && e.y == cur.x);
-                        if (!hit) {
-                            V3 c1 = poly[cur.x], c2 = poly[cur.y], e1 = poly[e.x], e2 = poly[e.y];
-                            hit = (veq(c1, e1) && veq(c2, e2)) || (veq(c1, e2) && veq(c2, e1));
-                        }
-                        if (hit) {
-                            --ne;
-                            edges[j] = edges[ne];
-                            removed = true;
-                            break;
-                        }
-                    }
-                    if (!removed) {
-                        if (ne == EPA_MAXE) { err |= ERR_EPA_CAPACITY; return -1; }
-                        edges[ne++] = e;
-                    }
-                }
+                if (!add_edge(make_int2(f.x, f.y), err) || !add_edge(make_int2(f.y, f.z), err) ||
+                    !add_edge(make_int2(f.z, f.x), err))
+                    return -1;
                 // swap-remove faces / distances / normals together (epa.cpp:191-195)
                 --nf;
                 faces[i] = faces[nf];
@@ -549,6 +573,14 @@ struct EpaPoly {
     }
 };
 
+// Orders the memory accesses of a lane group (one wave): writes before the
+// call are seen by the group's reads after it.
+__device__ __forceinline__ void group_mem_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 // Value of lane 0 of the caller's aligned group of G lanes.
 template <int G> __device__ __forceinline__ int group_bcast(int v) { return __shfl(v, 0, G); }
 template <int G> __device__ __forceinline__ double group_bcast(double v) { return __shfl(v, 0, G); }
@@ -578,6 +610,144 @@ RP_HD void epa_bind(EpaPoly& P, char* m) {
     P.faces = (I3*)m;
 }
 
+// Visibility flags of up to EPA_MAXF = 256 faces in four words.
+struct FaceBits {
+    unsigned long long w0 = 0, w1 = 0, w2 = 0, w3 = 0;
+    __device__ bool get(int i) const {
+        unsigned long long w = i < 64 ? w0 : i < 128 ? w1 : i < 192 ? w2 : w3;
+        return (w >> (i & 63)) & 1ull;
+    }
+    __device__ void put(int i, bool v) {
+        unsigned long long b = 1ull << (i & 63);
+        int k = i >> 6;
+        if (k == 0) w0 = v ? (w0 | b) : (w0 & ~b);
+        else if (k == 1) w1 = v ? (w1 | b) : (w1 & ~b);
+        else if (k == 2) w2 = v ? (w2 | b) : (w2 & ~b);
+        else w3 = v ? (w3 | b) : (w3 & ~b);
+    }
+    __device__ void or_bits(int c0, unsigned long long m) {  // c0 a multiple of the group size
+        unsigned long long b = m << (c0 & 63);
+        int k = c0 >> 6;
+        w0 |= k == 0 ? b : 0ull;
+        w1 |= k == 1 ? b : 0ull;
+        w2 |= k == 2 ? b : 0ull;
+        w3 |= k == 3 ? b : 0ull;
+    }
+};
+// epa for a lane group (G > 1), with the results of EpaPoly::step: the
+// polytope lives in the group's working memory; the lanes test which faces
+// see the support point, compute the new faces' normals and distances, and
+// search the closest face (first minimum), while lane 0 keeps the edge list
+// and the face list's swap-remove order (epa.cpp:165-195). Every lane
+// returns the status and the result; the EPA error bits are lane 0's.
+template <int G, class View>
+__device__ int epa_group(const View& h1, const View& h2, const Simplex& s, V3& n_out, double& pen_out, uint32_t& err,
+                         char* work) {
+    EpaPoly P;
+    epa_bind(P, work);
+    int gl = (int)(__lane_id() & (G - 1));
+    bool leader = gl == 0;
+    int gbase = (int)__lane_id() & ~(G - 1);
+    int st = 2;
+    if (leader && !P.init(s, err)) st = -1;
+    st = group_bcast<G>(st);
+    if (st != 2) return st;
+    V3 min_n = group_bcast<G>(P.min_n);
+    double min_d = group_bcast<G>(P.min_d);
+    int np = 4, nf = 4;
+    group_mem_sync();  // lane 0's initial faces
+    for (int it = 0; it < 100; ++it) {
+        V3 sp = support_md<G>(h1, h2, min_n, err);
+        double d = dot(min_n, sp);
+        if (fabs(d - min_d) < 0.0001) {
+            n_out = min_n;
+            pen_out = min_d;
+            return 1;
+        }
+        int new_idx = np;
+        if (leader) P.poly[np] = sp;
+        ++np;
+        // which faces see sp (a property of each face: computed up front)
+        FaceBits vis;
+        for (int c0 = 0; c0 < nf; c0 += G) {
+            int i = c0 + gl;
+            bool v = i < nf && P.sees(i, sp);
+            unsigned long long m = (__ballot(v) >> gbase) & ((1ull << G) - 1ull);
+            vis.or_bits(c0, m);
+        }
+        // lane 0: edges of the visible faces and the swap-removes, in order
+        int ne = 0;
+        if (leader) {
+            P.nf = nf;
+            P.ne = 0;
+            for (int i = 0; i < P.nf; ++i) {
+                if (vis.get(i)) {
+                    I3 f = P.faces[i];
+                    if (!P.add_edge(make_int2(f.x, f.y), err) || !P.add_edge(make_int2(f.y, f.z), err) ||
+                        !P.add_edge(make_int2(f.z, f.x), err)) {
+                        st = -1;
+                        break;
+                    }
+                    --P.nf;
+                    P.faces[i] = P.faces[P.nf];
+                    P.dist[i] = P.dist[P.nf];
+                    P.normals[i] = P.normals[P.nf];
+                    vis.put(i, vis.get(P.nf));
+                    --i;
+                }
+            }
+            nf = P.nf;
+            ne = P.ne;
+        }
+        st = group_bcast<G>(st);
+        if (st != 2) return st;
+        nf = group_bcast<G>(nf);
+        ne = group_bcast<G>(ne);
+        group_mem_sync();
+        // the new faces (edge i, new point) at nf + i; the sequential loop
+        // stops at the first full list or degenerate face, in that order
+        int room = EPA_MAXF - nf;
+        int nnew = ne < room ? ne : room;
+        int first_bad = nnew;
+        for (int i = gl; i < nnew; i += G) {
+            int2 e = P.edges[i];
+            I3 f = {e.x, e.y, new_idx};
+            V3 n;
+            double fd;
+            bool ok = epa_face(f, P.poly, np, n, fd);
+            P.faces[nf + i] = f;
+            P.normals[nf + i] = n;
+            P.dist[nf + i] = fd;
+            if (!ok && i < first_bad) first_bad = i;
+        }
+#pragma unroll
+        for (int off = 1; off < G; off <<= 1) first_bad = min(first_bad, __shfl_xor(first_bad, off, G));
+        if (first_bad < nnew) {
+            if (leader) err |= ERR_EPA_DEGENERATE;
+            return -1;
+        }
+        if (ne > room) {
+            if (leader) err |= ERR_EPA_CAPACITY;
+            return -1;
+        }
+        nf += ne;
+        group_mem_sync();
+        // the closest face: first minimum of the distances
+        double md = DBL_MAX;
+        int mi = -1;
+        for (int i = gl; i < nf; i += G) {
+            double di = P.dist[i];
+            bool t = di < md;
+            md = t ? di : md;
+            mi = t ? i : mi;
+        }
+        group_argmin<G>(md, mi);
+        min_d = md;
+        if (mi >= 0) min_n = P.normals[mi];
+    }
+    return 0;
+}
+
 // epa (epa.cpp:118-237): 0 = no convergence, 1 = converged, -1 = error.
 // G > 1: the group's lanes compute each support point together; the
 // polytope lives in lane 0 alone (in the group's working memory `work`),
@@ -603,37 +773,12 @@ RP_HD int epa(const View& h1, const View& h2, const Simplex& s, V3& n_out, doubl
         }
         return 0;
     } else {
-        EpaPoly P;
-        epa_bind(P, work);
-        bool leader = (__lane_id() & (G - 1)) == 0;
-        int st = 2;
-        V3 dir = v3(0.0, 0.0, 0.0);
-        if (leader) {
-            if (!P.init(s, err)) st = -1;
-            dir = P.min_n;
-        }
-        st = group_bcast<G>(st);
-        if (st != 2) return st;
-        dir = group_bcast<G>(dir);
-        for (int it = 0; it < 100; ++it) {
-            V3 sp = support_md<G>(h1, h2, dir, err);
-            if (leader) {
-                st = P.step(sp, n_out, pen_out, err);
-                dir = P.min_n;
-            }
-            st = group_bcast<G>(st);
-            if (st == 1) {
-                n_out = group_bcast<G>(n_out);
-                pen_out = group_bcast<G>(pen_out);
-            }
-            if (st != 2) return st;
-            dir = group_bcast<G>(dir);
-        }
-        return 0;
+        return epa_group<G>(h1, h2, s, n_out, pen_out, err, work);
     }
 }
 
 // ---------------------------------------------------------------------------
+// Contact manifold (clipping.cpp:7-371)// ---------------------------------------------------------------------------
 // Contact manifold (clipping.cpp:7-371)
 // ---------------------------------------------------------------------------
 struct Plane { V3 normal, point; };
@@ -688,11 +833,6 @@ RP_HD int clip_pass(const V3* in, int n, const Plane& p, V3* out, bool remove, u
 // every lane one edge (start = in[j - 1], end = in[j]); a lane's 0-2 output
 // points go to their place in the sequential output order by a prefix sum
 // over the group, so the polygon equals clip_pass's point for point.
-__device__ __forceinline__ void group_mem_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
 template <int G> __device__ __forceinline__ int group_inclusive_sum(int v) {
     int gl = (int)(__lane_id() & (G - 1));
 #pragma unroll
