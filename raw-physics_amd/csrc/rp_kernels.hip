@@ -389,13 +389,13 @@ __device__ __forceinline__ void gjk_epa_pairs(DevWorld& w, const int32_t* list, 
                 int used = h1.nv + h2.nv;  // the rest of the pair's LDS slice is free
                 nc = manifold<G>(h1, h2, n, cts, err, work, st + used, G > 1 ? stage - used : 0);
             }
-            if (gl == 0) {
-                const Body& e1 = w.body[b1];
-                const Body& e2 = w.body[b2];
-                Contact* out = w.contacts + (size_t)g * MAXC;
-                for (int c = 0; c < nc; ++c) contact_to_constraint(e1, e2, cts[c], out[c]);
-                w.ncontact[g] = nc;
-            }
+            // one contact per lane (the group's count is uniform)
+            if constexpr (G > 1) group_lds_sync();
+            const Body& e1 = w.body[b1];
+            const Body& e2 = w.body[b2];
+            Contact* out = w.contacts + (size_t)g * MAXC;
+            for (int c = gl; c < nc; c += G) contact_to_constraint(e1, e2, cts[c], out[c]);
+            if (gl == 0) w.ncontact[g] = nc;
         }
         if (gl == 0) {
             if constexpr (!kManifold) {
@@ -444,11 +444,12 @@ __global__ void __launch_bounds__(64) k_manifold(DevWorld w) {
         ContactPt* cts = G == 1 ? local : (ContactPt*)(work + CLIP_BYTES);
         int used = h1.nv + h2.nv;
         int nc = manifold<G>(h1, h2, w.pair_normal[g], cts, err, work, st + used, G > 1 ? w.stage_cap - used : 0);
+        if constexpr (G > 1) group_lds_sync();
+        const Body& e1 = w.body[b1];
+        const Body& e2 = w.body[b2];
+        Contact* out = w.contacts + (size_t)g * MAXC;
+        for (int c = gl; c < nc; c += G) contact_to_constraint(e1, e2, cts[c], out[c]);
         if (gl == 0) {
-            const Body& e1 = w.body[b1];
-            const Body& e2 = w.body[b2];
-            Contact* out = w.contacts + (size_t)g * MAXC;
-            for (int c = 0; c < nc; ++c) contact_to_constraint(e1, e2, cts[c], out[c]);
             w.ncontact[g] = nc;
             if (err) atomicOr(&w.scene_err[s], err);
         }

@@ -1022,9 +1022,55 @@ RP_HD int contacts_from_clip(const V3* pts, int n, const Plane& rp, V3 normal, b
     return nc;
 }
 
+// contacts_from_clip on the group's lanes: lane j takes point j; the points
+// kept (negative penetration) go to `out` in point order by a prefix sum,
+// the first MAXC of them (more set ERR_CONTACT_CAPACITY, as in order).
+// Returns the count on every lane of the group.
+template <int G>
+__device__ int contacts_from_clip_group(const V3* pts, int n, const Plane& rp, V3 normal, bool ref1, ContactPt* out,
+                                        uint32_t& err) {
+    int gl = (int)(__lane_id() & (G - 1));
+    int nc = 0;
+    double pd = dot(scl(-1.0, rp.normal), rp.point);
+    for (int c0 = 0; c0 < n; c0 += G) {
+        int i = c0 + gl;
+        bool keep = false;
+        ContactPt c;
+        if (i < n) {
+            V3 pt = pts[i];
+            V3 cp = sub(pt, scl(dot(rp.normal, pt) + pd, rp.normal));
+            V3 diff = sub(pt, cp);
+            double pen;
+            if (ref1) {
+                pen = dot(diff, normal);
+                c.p1 = sub(pt, scl(pen, normal));
+                c.p2 = pt;
+            } else {
+                pen = -dot(diff, normal);
+                c.p1 = pt;
+                c.p2 = add(pt, scl(pen, normal));
+            }
+            c.n = normal;
+            keep = pen < 0.0;
+        }
+        int k = keep ? 1 : 0;
+        int incl = group_inclusive_sum<G>(k);
+        int total = __shfl(incl, G - 1, G);
+        int pos = nc + incl - k;
+        if (keep && pos < MAXC) out[pos] = c;
+        nc += total;
+    }
+    if (nc > MAXC) {
+        if (gl == 0) err |= ERR_CONTACT_CAPACITY;
+        nc = MAXC;
+    }
+    return nc;
+}
+
 // G > 1: the group clips in `spare` (spare_n points: the unused end of the
 // pair's LDS slice) when both polygons fit there, else lane 0 clips in
-// `work`; the contact points are lane 0's.
+// `work`. The contact points go to `out`; every lane of the group returns
+// their count.
 template <int G = 1>
 RP_HD int manifold(const HullView& h1, const HullView& h2, V3 normal, ContactPt* out, uint32_t& err,
                    char* work = nullptr, V3* spare = nullptr, int spare_n = 0) {
@@ -1104,10 +1150,10 @@ RP_HD int manifold(const HullView& h1, const HullView& h2, V3 normal, ContactPt*
         V3 dd1 = sub(h1.tv[e1b], p1);
         V3 p2 = h2.tv[e2a];
         V3 dd2 = sub(h2.tv[e2b], p2);
-        if constexpr (G > 1) {
-            if ((__lane_id() & (G - 1)) != 0) return 0;  // lane 0 writes the contact
-        }
         if (!skew_lines(p1, dd1, p2, dd2, l1, l2)) { err |= ERR_SKEW_LINES; return 0; }
+        if constexpr (G > 1) {
+            if ((__lane_id() & (G - 1)) != 0) return 1;  // lane 0 writes the contact
+        }
         out[0].p1 = l1; out[0].p2 = l2; out[0].n = normal;
         return 1;
     }
@@ -1120,21 +1166,23 @@ RP_HD int manifold(const HullView& h1, const HullView& h2, V3 normal, ContactPt*
     rp.normal = inv3(face_normal_world(rh, rf));
     rp.point = rh.tv[rh.fe[rh.fe_off[rf]]];
     const V3* outp = nullptr;
-    int n = -1;
     if constexpr (G > 1) {
         int cap = spare_n / 2 < CLIP_MAX ? spare_n / 2 : CLIP_MAX;
+        int n = -1;
         if (cap >= 8) n = clip_incident_group<G>(rh, rf, ih, inf, rp, spare, spare + cap, cap, outp);
-        if ((__lane_id() & (G - 1)) != 0) return 0;  // lane 0 goes on
+        if (n >= 0) return contacts_from_clip_group<G>(outp, n, rp, normal, ref1, out, err);
+        // lane 0 clips in the group's working memory
+        int nc = 0;
+        if ((__lane_id() & (G - 1)) == 0) {
+            int m = clip_serial(rh, rf, ih, inf, rp, (V3*)work, (V3*)work + CLIP_MAX, outp, err);
+            nc = contacts_from_clip(outp, m, rp, normal, ref1, out, err);
+        }
+        return group_bcast<G>(nc);
+    } else {
+        V3 localA[CLIP_MAX], localB[CLIP_MAX];  // here, in the frame that reads the result
+        int n = clip_serial(rh, rf, ih, inf, rp, localA, localB, outp, err);
+        return contacts_from_clip(outp, n, rp, normal, ref1, out, err);
     }
-    // one lane: private arrays (G = 1) or the group's working memory; the
-    // arrays live here, in the frame that reads the result
-    V3 localA[G == 1 ? CLIP_MAX : 1], localB[G == 1 ? CLIP_MAX : 1];
-    if (n < 0) {
-        V3* bufA = G == 1 ? localA : (V3*)work;
-        V3* bufB = G == 1 ? localB : (V3*)work + CLIP_MAX;
-        n = clip_serial(rh, rf, ih, inf, rp, bufA, bufB, outp, err);
-    }
-    return contacts_from_clip(outp, n, rp, normal, ref1, out, err);
 }
 
 // collider_get_contacts for two hulls (collider.cpp:523-558): GJK, EPA, manifold.
