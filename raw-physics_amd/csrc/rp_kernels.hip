@@ -187,11 +187,8 @@ __global__ void k_scatter_live(DevWorld w) {
 }
 
 // ---------------------------------------------------------------------------
-__global__ void k_integrate(DevWorld w, double h) {
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b == 0) w.counters[CNT_MANIFOLD] = 0;  // the substep's manifold queue starts empty
-    if (b >= w.nb) return;
-    b += w.body_base;
+// Integration of body b (pbd.cpp:565-596) and its model matrix.
+__device__ __forceinline__ void integrate_body(const DevWorld& w, int b, double h) {
     Body e = w.body[b];
     integrate(e, h);
     w.body[b].x = e.x;
@@ -201,6 +198,13 @@ __global__ void k_integrate(DevWorld w, double h) {
     w.body[b].px = e.px;
     w.body[b].pq = e.pq;
     w.model[b] = model_matrix(e.x, e.q);
+}
+
+__global__ void k_integrate(DevWorld w, double h) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b == 0) w.counters[CNT_MANIFOLD] = 0;  // the substep's manifold queue starts empty
+    if (b >= w.nb) return;
+    integrate_body(w, b + w.body_base, h);
 }
 
 // The pose every body has when the substep's collision pass reads it
@@ -578,7 +582,7 @@ __global__ void __launch_bounds__(64) k_levels(DevWorld w, int collisions) {
 // solves 64/LPS scenes side by side (small scenes leave most of a 64-lane
 // wave idle otherwise).
 template <bool kExt, int LPS>
-__global__ void __launch_bounds__(64) k_solve(DevWorld w, double h, int iters, int collisions) {
+__global__ void __launch_bounds__(64) k_solve(DevWorld w, double h, int iters, int collisions, int integrate_next) {
     constexpr int SPW = 64 / LPS;
     int s = blockIdx.x * SPW + (int)threadIdx.x / LPS;
     int lane = (int)threadIdx.x % LPS;
@@ -628,17 +632,24 @@ __global__ void __launch_bounds__(64) k_solve(DevWorld w, double h, int iters, i
         __syncthreads();
     }
     if (err && valid) atomicOr(&w.scene_err[s], err);
+    // the next substep's integration of the scene's bodies (worlds without
+    // large scenes), in place of a k_integrate launch
+    if (integrate_next) {
+        __syncthreads();
+        for (int i = lane; i < n; i += LPS) integrate_body(w, b0 + i, h);
+        if (blockIdx.x == 0 && threadIdx.x == 0) w.counters[CNT_MANIFOLD] = 0;
+    }
 }
 
 template <bool kExt>
-static void launch_solve(const DevWorld& w, const FrameParams& fp, hipStream_t st) {
+static void launch_solve(const DevWorld& w, const FrameParams& fp, hipStream_t st, int integrate_next) {
     int spw = 64 / w.solve_lanes;
     dim3 grid((w.ns + spw - 1) / spw);
     int it = (int)fp.iters, co = fp.collisions ? 1 : 0;
     switch (w.solve_lanes) {
-    case 16: hipLaunchKernelGGL((k_solve<kExt, 16>), grid, dim3(64), 0, st, w, fp.h, it, co); break;
-    case 32: hipLaunchKernelGGL((k_solve<kExt, 32>), grid, dim3(64), 0, st, w, fp.h, it, co); break;
-    default: hipLaunchKernelGGL((k_solve<kExt, 64>), grid, dim3(64), 0, st, w, fp.h, it, co); break;
+    case 16: hipLaunchKernelGGL((k_solve<kExt, 16>), grid, dim3(64), 0, st, w, fp.h, it, co, integrate_next); break;
+    case 32: hipLaunchKernelGGL((k_solve<kExt, 32>), grid, dim3(64), 0, st, w, fp.h, it, co, integrate_next); break;
+    default: hipLaunchKernelGGL((k_solve<kExt, 64>), grid, dim3(64), 0, st, w, fp.h, it, co, integrate_next); break;
     }
 }
 
@@ -723,11 +734,16 @@ hipError_t launch_frame(const DevWorld& w, const FrameParams& fp, hipStream_t st
     hipLaunchKernelGGL(k_levels, dim3(w.ns), dim3(64), 0, st, w, fp.collisions ? 1 : 0);
     for (int k = 0; k < nbig; ++k) big_colour(w, bigs[k], fp.collisions, st);
     t.end(st);
+    // without large scenes or recording, the solve kernel integrates its
+    // scenes' bodies for the next substep
+    bool fuse_integrate = nbig == 0 && !rec;
     for (unsigned sub = 0; sub < fp.substeps; ++sub) {
         if (rec) hipLaunchKernelGGL(k_rec_bodies, dim3(cdiv(w.nb, 256)), dim3(256), 0, st, w, *rec, (int)sub);
-        t.begin(KC_INTEGRATE, st);
-        hipLaunchKernelGGL(k_integrate, dim3(cdiv(w.nb, 128)), dim3(128), 0, st, w, fp.h);
-        t.end(st);
+        if (sub == 0 || !fuse_integrate) {
+            t.begin(KC_INTEGRATE, st);
+            hipLaunchKernelGGL(k_integrate, dim3(cdiv(w.nb, 128)), dim3(128), 0, st, w, fp.h);
+            t.end(st);
+        }
         if (fp.keep_pose && sub + 1 == fp.substeps)
             hipLaunchKernelGGL(k_keep_pose, dim3(cdiv(w.nb, 256)), dim3(256), 0, st, w);
         if (fp.collisions) {
@@ -786,8 +802,9 @@ hipError_t launch_frame(const DevWorld& w, const FrameParams& fp, hipStream_t st
             if (rec) hipLaunchKernelGGL(k_rec_contacts, dim3(256), dim3(256), 0, st, w, *rec, (int)sub);
         }
         t.begin(KC_SOLVE, st);
-        if (w.n_ext > 0) launch_solve<true>(w, fp, st);
-        else launch_solve<false>(w, fp, st);
+        int next = fuse_integrate && sub + 1 < fp.substeps ? 1 : 0;
+        if (w.n_ext > 0) launch_solve<true>(w, fp, st, next);
+        else launch_solve<false>(w, fp, st, next);
         for (int k = 0; k < nbig; ++k) big_solve(w, bigs[k], fp, st);
         t.end(st);
     }
