@@ -119,12 +119,16 @@ def committed_traffic(kernel_class: str, schedule: dict):
     """HBM bytes per launch of a kernel from the committed rocprofv3 PMC
     summary (FETCH_SIZE and WRITE_SIZE passes, scaled by the copy-kernel
     calibration), when it covers this kernel under the run's schedules
-    ({"narrow": lanes, "manifold": lanes or 0, "solve": lanes}); else None."""
+    ({"narrow": lanes, "manifold": lanes or 0, "solve": lanes}) that shape
+    it (the narrow and manifold schedules for the narrow-phase kernels, the
+    solve schedule for the solve); else None."""
     if not TRAFFIC_FILE.exists():
         return None
     d = json.loads(TRAFFIC_FILE.read_text())
     k = d.get("kernels", {}).get(kernel_class)
-    if not k or d.get("schedule") != schedule:
+    shaping = {"narrow": ("narrow", "manifold"), "manifold": ("narrow", "manifold"), "solve": ("solve",)}
+    keys = shaping.get(kernel_class, tuple(schedule))
+    if not k or any(d.get("schedule", {}).get(key) != schedule.get(key) for key in keys):
         return None
     return {"bytes_per_launch": k["bytes_per_launch"], "source": f"{TRAFFIC_FILE.relative_to(ROOT)}: {d['window']}"}
 
