@@ -262,6 +262,8 @@ def main():
     ap.add_argument("--solve-lanes", type=int, default=None,
                     help="fixed solve schedule (lanes per scene); skips the autotuner")
     ap.add_argument("--eager", action="store_true", help="launch each kernel eagerly instead of replaying frame graphs")
+    ap.add_argument("--views", type=int, default=0,
+                    help="concurrent scene-range views for graph replay (0 = the library's automatic choice)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurement")
     ap.add_argument("--adjoint-scenes", type=int, default=2048)
     ap.add_argument("--adjoint-frames", type=int, default=120)
@@ -307,6 +309,8 @@ def main():
     tune = None
     if args.eager:
         gw.set_graphs(False)
+    if args.views:
+        gw.set_views(args.views)
     fixed = [a is not None for a in (args.narrow_lanes, args.manifold_lanes, args.solve_lanes)]
     if args.narrow_lanes is not None:
         gw.set_narrow_group(args.narrow_lanes)
