@@ -67,6 +67,24 @@ def cube_corners() -> np.ndarray:
     return _CUBE
 
 
+def prism_corners(n: int, radius: float = 1.0, half_height: float = 0.5) -> np.ndarray:
+    """Corner stream of a right prism over a regular n-gon (2n vertices): the
+    caps as triangle fans, which the hull builder merges into two n-vertex
+    faces, and each side as two triangles."""
+    a = [2.0 * math.pi * k / n for k in range(n)]
+    verts = [(radius * math.cos(t), half_height, radius * math.sin(t)) for t in a]
+    verts += [(radius * math.cos(t), -half_height, radius * math.sin(t)) for t in a]
+    faces = []
+    for k in range(1, n - 1):
+        faces.append((1, k + 2, k + 1))              # top cap, normal +y
+        faces.append((n + 1, n + k + 1, n + k + 2))  # bottom cap, normal -y
+    for k in range(n):
+        k1 = (k + 1) % n
+        faces.append((k + 1, k1 + 1, n + k1 + 1))
+        faces.append((k + 1, n + k1 + 1, n + k + 1))
+    return _corners(verts, faces)
+
+
 _ICO_CACHE = {}
 
 
@@ -716,4 +734,37 @@ def batched_uneven(num_scenes: int, first_seed: int = 0, **kw) -> World:
     w = World(name=f"batched_uneven_{num_scenes}")
     for s in range(num_scenes):
         uneven_scene(w, first_seed + s, **kw)
+    return w
+
+
+def prisms_on_pillars(num_scenes: int, first_seed: int = 0) -> World:
+    """Octagonal prisms (16 vertices, circumradius 0.47) dropped onto fixed
+    cube pillars (half-size 0.4), half of them level and turned 0-8 degrees
+    about y, half tilted 2-8 degrees, plus a decagonal prism (20 vertices) on
+    the floor. The prisms come first in body order, so a level prism's
+    octagon is the incident face clipped by the pillar's square, whose edges
+    cut all eight corners: the polygon grows to 10-12 points in the clipping
+    passes."""
+    w = World(name=f"prisms_{num_scenes}")
+    for sc in range(num_scenes):
+        rng = Rng(0x5EED0000 + first_seed + sc)
+        _floor(w)
+        oct_ = w.shape(prism_corners(8, 0.47, 0.2))
+        dec = w.shape(prism_corners(10, 0.6, 0.3))
+        cube = w.shape(cube_corners())
+        for k in range(4):
+            x, z = 3.0 * (k % 2) - 1.5, 3.0 * (k // 2) - 1.5
+            if k % 2 == 0:
+                q = quaternion_new((0.0, 1.0, 0.0), rng.uniform(0.0, 8.0))
+            else:
+                q = quaternion_new((rng.uniform(-1.0, 1.0), 0.0, rng.uniform(-1.0, 1.0)), rng.uniform(2.0, 8.0))
+            w.add_body(oct_, (1.0, 1.0, 1.0), (x + rng.uniform(-0.03, 0.03), 0.9 + rng.uniform(0.0, 0.3),
+                                                z + rng.uniform(-0.03, 0.03)), q,
+                       mass=hull_volume(w.shapes[oct_], (1.0, 1.0, 1.0)))
+        w.add_body(dec, (1.0, 1.0, 1.0), (0.0, 0.0, 4.0), quaternion_new((1.0, 0.0, 0.0), 10.0),
+                   mass=hull_volume(w.shapes[dec], (1.0, 1.0, 1.0)))
+        for k in range(4):
+            x, z = 3.0 * (k % 2) - 1.5, 3.0 * (k // 2) - 1.5
+            w.add_body(cube, (0.4, 0.4, 0.4), (x, 0.0, z), quaternion_new((0.0, 1.0, 0.0), 0.0), fixed=True)
+        w.end_scene()
     return w

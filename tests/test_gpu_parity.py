@@ -259,6 +259,26 @@ def test_size_classes_change_nothing():
     assert not bad, f"{big.name} against the oracle: {bad}"
 
 
+@pytest.mark.parametrize("lanes,manifold", [(8, 0), (16, 0), (8, 1)])
+def test_prisms_on_pillars_bit_exact(lanes, manifold):
+    """Octagonal prisms resting on cube pillars: with 16 + 8-vertex pairs in
+    slices sized for two 20-vertex hulls, the group clips in 8-point LDS
+    buffers; a level prism's octagon clipped by the pillar's square grows to
+    10-12 points, so the clipping is redone by lane 0 in working memory. The
+    contacts stay bit-exact on every schedule."""
+    world = scenes.prisms_on_pillars(6, first_seed=60)
+    o, g = SimWorld("orc", world), SimWorld("rp", world)
+    assert g.lib.rp_world_set_narrow_group(g.h, lanes) == 0
+    assert g.lib.rp_world_set_manifold_group(g.h, manifold) == 0
+    for done in range(0, 90, 30):
+        o.step(30)
+        g.step(30)
+        bad = state_bits_equal(o.state(), g.state())
+        assert not bad, f"{lanes}/{manifold}, frame {done + 30}: {bad}"
+    for s in range(world.num_scenes):
+        assert g.error(s) == 0
+
+
 def _large_beside_small():
     w = scenes.large_scene(300, seed=1, large_scene_bodies=64)
     for seed in range(3):

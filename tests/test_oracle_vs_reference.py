@@ -33,6 +33,12 @@ def test_joint_chains_bit_exact():
     _compare(scenes.chains(2, length=5, first_seed=78), 200, every=50, collisions=False, substeps=50, iters=50)
 
 
+def test_prisms_on_pillars_bit_exact():
+    """Octagonal and decagonal prisms (8- and 10-vertex cap faces) resting on
+    cube pillars: clipping polygons of up to a dozen points."""
+    _compare(scenes.prisms_on_pillars(3, first_seed=60), 90, every=30)
+
+
 def test_substep_and_iteration_counts():
     w = scenes.batched_uniform(2, first_seed=5)
     _compare(w, 30, every=10, substeps=1, iters=1)
