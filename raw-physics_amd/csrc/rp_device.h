@@ -26,10 +26,13 @@ __device__ __forceinline__ void load_vel_body(const Body& g, Body& b) {
 }
 
 // The broad-phase test of two bodies (broad.cpp:17-21), body i first.
-__device__ __forceinline__ bool broad_hit(const Body& e1, const Body& e2) {
-    double dd = len(sub(e1.x, e2.x));
-    double maxd = e1.radius + e2.radius + 0.1;
+__device__ __forceinline__ bool broad_hit(V3 x1, double r1, V3 x2, double r2) {
+    double dd = len(sub(x1, x2));
+    double maxd = r1 + r2 + 0.1;
     return dd <= maxd;
+}
+__device__ __forceinline__ bool broad_hit(const Body& e1, const Body& e2) {
+    return broad_hit(e1.x, e1.radius, e2.x, e2.radius);
 }
 
 // One solve item: an external constraint (item < 0, index ~item) or the

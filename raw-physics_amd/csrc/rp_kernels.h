@@ -114,6 +114,7 @@ struct BigScene {
     int32_t* cell_start;         // [T + 1]
     int32_t* cell_cursor;        // [T]
     int32_t* sorted;             // [n] small bodies grouped by bucket
+    double4* spos;               // [n] their positions and radii, in the same order
     int32_t* pair_off;           // [n + 1] per-body pair counts, then offsets
     int32_t* tmp;                // [max(pair_cap, item_cap) + 1] scan buffer
     int32_t* bi_start;           // [n + 1] body -> items CSR

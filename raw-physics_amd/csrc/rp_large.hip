@@ -149,9 +149,11 @@ __global__ void k_big_count(DevWorld w, BigScene b) {
 __global__ void k_big_fill(DevWorld w, BigScene b) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < b.n; i += gridDim.x * blockDim.x) {
         if (b.is_large[i]) continue;
-        Cell c = cell_of(w.body[b.b0 + i].x, b.cell);
+        const Body& e = w.body[b.b0 + i];
+        Cell c = cell_of(e.x, b.cell);
         int pos = atomicAdd(&b.cell_cursor[cell_hash(c.x, c.y, c.z, b.T)], 1);
         b.sorted[pos] = i;
+        b.spos[pos] = make_double4(e.x.x, e.x.y, e.x.z, e.radius);
     }
 }
 
@@ -163,7 +165,9 @@ __global__ void k_big_pairs_small(DevWorld w, BigScene b) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < b.n; i += gridDim.x * blockDim.x) {
         if (b.is_large[i]) continue;
         const Body& e1 = w.body[b.b0 + i];
-        Cell c = cell_of(e1.x, b.cell);
+        const V3 x1 = e1.x;
+        const double r1 = e1.radius;
+        Cell c = cell_of(x1, b.cell);
         int keys[27];
         int nk = 0, cnt = 0;
         int off = 0;
@@ -179,7 +183,8 @@ __global__ void k_big_pairs_small(DevWorld w, BigScene b) {
                     keys[nk++] = key;
                     for (int m = b.cell_start[key]; m < b.cell_start[key + 1]; ++m) {
                         int j = b.sorted[m];
-                        if (j <= i || !broad_hit(e1, w.body[b.b0 + j])) continue;
+                        double4 q = b.spos[m];  // the bucket's bodies, contiguous
+                        if (j <= i || !broad_hit(x1, r1, v3(q.x, q.y, q.z), q.w)) continue;
                         if (PASS == 1 && off + cnt < b.pair_cap) out[off + cnt] = make_int2(i, j);
                         ++cnt;
                     }

@@ -612,6 +612,7 @@ static bool build_image(const HostWorld& hw, DeviceImage& im) {
         ok &= im.alloc(bs.cell_start, (size_t)bs.T + 1);
         ok &= im.alloc(bs.cell_cursor, (size_t)bs.T);
         ok &= im.alloc(bs.sorted, (size_t)bs.n);
+        ok &= im.alloc(bs.spos, (size_t)bs.n);
         ok &= im.alloc(bs.pair_off, (size_t)bs.n + 1);
         ok &= im.alloc(bs.tmp, (size_t)tmp_n);
         ok &= im.alloc(bs.bi_start, (size_t)bs.n + 1);
