@@ -160,13 +160,16 @@ __global__ void k_big_fill(DevWorld w, BigScene b) {
 // Pairs (i, j > i) of small body i: small bodies in the 27 neighbouring
 // cells (each bucket visited once) and every large body. pass 0 counts into
 // pair_off[i]; pass 1 writes at pair_off[i] and sorts the segment by j.
+// Bodies are visited in bucket order, so that neighbouring lanes search the
+// same cells (each body's segment is its own, whatever the visiting order).
 template <int PASS>
 __global__ void k_big_pairs_small(DevWorld w, BigScene b) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < b.n; i += gridDim.x * blockDim.x) {
-        if (b.is_large[i]) continue;
-        const Body& e1 = w.body[b.b0 + i];
-        const V3 x1 = e1.x;
-        const double r1 = e1.radius;
+    int nsmall = b.cell_start[b.T];
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nsmall; t += gridDim.x * blockDim.x) {
+        int i = b.sorted[t];
+        double4 p1 = b.spos[t];
+        const V3 x1 = v3(p1.x, p1.y, p1.z);
+        const double r1 = p1.w;
         Cell c = cell_of(x1, b.cell);
         int keys[27];
         int nk = 0, cnt = 0;
@@ -191,7 +194,8 @@ __global__ void k_big_pairs_small(DevWorld w, BigScene b) {
                 }
         for (int k = 0; k < b.n_large; ++k) {
             int j = b.large[k];
-            if (j <= i || !broad_hit(e1, w.body[b.b0 + j])) continue;
+            const Body& e2 = w.body[b.b0 + j];
+            if (j <= i || !broad_hit(x1, r1, e2.x, e2.radius)) continue;
             if (PASS == 1 && off + cnt < b.pair_cap) out[off + cnt] = make_int2(i, j);
             ++cnt;
         }
