@@ -279,6 +279,22 @@ def test_prisms_on_pillars_bit_exact(lanes, manifold):
         assert g.error(s) == 0
 
 
+def test_long_run_default_schedule_bit_exact():
+    """600 frames of 64 scenes on the default path (three concurrent views,
+    graph replay, fused lane-group narrow phase, solve with integration):
+    falling, resting and sleeping bodies, bit for bit at frames 300 and 600."""
+    world = scenes.batched_uniform(64, first_seed=700)
+    o, g = SimWorld("orc", world), SimWorld("rp", world)
+    for done in (300, 600):
+        o.step(300)
+        g.step(300)
+        bad = state_bits_equal(o.state(), g.state())
+        assert not bad, f"frame {done}: {bad}"
+    assert g.lib.rp_world_views(g.h) == 3
+    for s in range(world.num_scenes):
+        assert g.error(s) == 0
+
+
 def _large_beside_small():
     w = scenes.large_scene(300, seed=1, large_scene_bodies=64)
     for seed in range(3):
