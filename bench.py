@@ -5,7 +5,11 @@ Workload (batched uniform): 4096 independent scenes, each a static 50x1x50
 floor plus 16 dynamic bodies (8 cubes with 8-vertex hulls, 8 icospheres with
 42-vertex hulls, size U(0.5, 1.0), mass = hull volume), dropped from a 4x4
 grid, dt = 1/60, 20 substeps, 1 position iteration, FP64, seeded synthetic
-inputs. One "step" is one frame of all 4096 scenes (20 substeps each).
+inputs, 600 frames. One "step" is one frame of all 4096 scenes (20 substeps
+each). The batch is first stepped untimed to --start-frame (300), then
+--steps frames are timed from there: a window deep in the run, with falling,
+resting and sleeping scenes, rather than the first frames' free fall. The
+whole 600-frame run from frame 0 is timed as well (`whole_run`).
 
 Metric: body-substeps/s = dynamic bodies x substeps x frames / second, for
   * the batched method (all scenes resident in HBM, one launch sequence per
@@ -23,7 +27,9 @@ import ctypes
 import importlib.util
 import json
 import os
+import subprocess
 import sys
+import tempfile
 import time
 from pathlib import Path
 
@@ -49,40 +55,118 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(scenes_mod, frames: int, budget_s: float):
-    """Times the reference step (oracle/_ref, compiled from the reference's own
-    sources) or, where that build is absent, the oracle's C restatement, on
-    one host core over a bounded sample of the same workload."""
-    sys.path.insert(0, str(ROOT / "tests"))
-    from _libs import SimWorld, load  # test infrastructure: the CPU checker only
+def _scene_slice(world, st: dict, first: int, count: int) -> dict:
+    """The state rows of scenes [first, first + count) of a world."""
+    off = world.scene_body_offset
+    a, b = off[first], off[first + count]
+    return {k: np.ascontiguousarray(v[a:b]) for k, v in st.items()}
 
-    kind = "ref" if load("ref") is not None else "orc"
-    # estimate cost on one scene, then size the sample to the time budget
-    probe = SimWorld(kind, scenes_mod.batched_uniform(1, first_seed=0))
+
+def cpu_worker(args):
+    """One process of the all-cores CPU leg: builds its scenes in the given
+    reference build, loads their window-start state, reports ready, waits for
+    the start line on stdin, steps its frames and prints the elapsed time."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from _libs import SimWorld  # test infrastructure: the CPU reference only
+
+    sc = _package().scenes
+    with np.load(args.cpu_worker, allow_pickle=False) as z:
+        st = {k: z[k] for k in z.files}
+    world = sc.batched_uniform(args.worker_scenes, first_seed=args.worker_first)
+    sim = SimWorld(args.worker_kind, world)
+    sim.set_state(st)
+    print("ready", flush=True)
+    sys.stdin.readline()
     t0 = time.perf_counter()
-    probe.step(min(frames, 60))
-    per_scene_frame = (time.perf_counter() - t0) / min(frames, 60)
-    probe.close()
-    n_scenes = max(1, min(64, int(budget_s / (per_scene_frame * frames))))
-    world = scenes_mod.batched_uniform(n_scenes, first_seed=0)
-    sim = SimWorld(kind, world)
-    t0 = time.perf_counter()
-    sim.step(frames)
-    dt = time.perf_counter() - t0
-    dyn = world.dynamic_bodies()
-    return {"value": dyn * 20 * frames / dt, "unit": "body-substeps/s", "cores": 1,
-            "kind": "reference" if kind == "ref" else "port",
-            "sample": f"{n_scenes} of the 4096 scenes (seeds 0..{n_scenes - 1}) x {frames} frames, "
-                      f"single thread, {dt:.1f} s"}
+    sim.step(args.steps)
+    print(json.dumps({"elapsed": time.perf_counter() - t0}), flush=True)
+
+
+def cpu_baseline(scenes_mod, world, start_state: dict, start_frame: int, frames: int, budget_s: float):
+    """Times the reference step on the host over a bounded sample of the
+    timed window: the first N scenes of the batch, from the batch's
+    window-start state (the GPU state is bit-identical to the reference's),
+    for `frames` frames. Three legs: the reference compiled from its sources
+    at the parity flags on one core, at its release flags (-O2 -ffast-math)
+    on one core, and at its release flags on every host core available (up
+    to 16), one process per core on its own scenes (the reference step is
+    single-threaded and not reentrant). The fastest is the baseline.
+    Falls back to the oracle's C restatement where the reference build is
+    absent."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from _libs import SimWorld, load  # test infrastructure: the CPU reference only
+
+    kinds = [k for k in ("ref", "ref_fast") if load(k) is not None] or ["orc"]
+    legs = {}
+    per_scene_frame = None
+    for kind in kinds:
+        probe_world = scenes_mod.batched_uniform(4, first_seed=0)
+        probe = SimWorld(kind, probe_world)
+        probe.set_state(_scene_slice(world, start_state, 0, 4))
+        t0 = time.perf_counter()
+        probe.step(min(frames, 5))
+        per_scene_frame = (time.perf_counter() - t0) / min(frames, 5) / 4
+        probe.close()
+        n_scenes = max(1, min(world.num_scenes, int(budget_s / (per_scene_frame * frames))))
+        w = scenes_mod.batched_uniform(n_scenes, first_seed=0)
+        sim = SimWorld(kind, w)
+        sim.set_state(_scene_slice(world, start_state, 0, n_scenes))
+        t0 = time.perf_counter()
+        sim.step(frames)
+        dt = time.perf_counter() - t0
+        sim.close()
+        legs[kind] = {"value": w.dynamic_bodies() * 20 * frames / dt, "cores": 1, "scenes": n_scenes,
+                      "seconds": dt}
+    fast = "ref_fast" if "ref_fast" in kinds else kinds[-1]
+    cores = min(16, len(os.sched_getaffinity(0)))
+    if cores > 1:
+        # per process the same sample size as the one-core leg of this build
+        n_each = min(legs[fast]["scenes"], world.num_scenes // cores)
+        with tempfile.TemporaryDirectory() as tmp:
+            procs = []
+            for c in range(cores):
+                path = Path(tmp) / f"state{c}.npz"
+                np.savez(path, **_scene_slice(world, start_state, c * n_each, n_each))
+                procs.append(subprocess.Popen(
+                    [sys.executable, str(ROOT / "bench.py"), "--cpu-worker", str(path), "--worker-kind", fast,
+                     "--worker-first", str(c * n_each), "--worker-scenes", str(n_each), "--steps", str(frames)],
+                    stdin=subprocess.PIPE, stdout=subprocess.PIPE, text=True))
+            for pr in procs:
+                assert pr.stdout.readline().strip() == "ready"
+            t0 = time.perf_counter()
+            for pr in procs:
+                pr.stdin.write("go\n")
+                pr.stdin.flush()
+            el = [json.loads(pr.stdout.readline())["elapsed"] for pr in procs]
+            wall = time.perf_counter() - t0
+            for pr in procs:
+                pr.wait()
+        body_substeps = cores * n_each * 16 * 20 * frames
+        legs[fast + "_all_cores"] = {"value": body_substeps / wall, "cores": cores, "scenes": cores * n_each,
+                                     "seconds": wall, "slowest_process_s": max(el)}
+    best = max(legs, key=lambda k: legs[k]["value"])
+    b = legs[best]
+    names = {"ref": "the reference at its parity flags (-O2 -fno-fast-math -ffp-contract=off)",
+             "ref_fast": "the reference at its release flags (-O2 -ffast-math)",
+             "orc": "the oracle's C restatement (parity flags)"}
+    base = best.replace("_all_cores", "")
+    return {"value": b["value"], "unit": "body-substeps/s", "cores": b["cores"],
+            "kind": "port" if base == "orc" else "reference",
+            "sample": f"{names[base]}, {b['scenes']} of the {world.num_scenes} scenes (seeds 0..{b['scenes'] - 1}) "
+                      f"x frames {start_frame}-{start_frame + frames - 1} from the batch's frame-{start_frame} "
+                      f"state, {b['cores']} process(es), {b['seconds']:.1f} s",
+            "legs": legs}
 
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (and matrix) peak
 
 
-def op_counts(scenes_mod, n_scenes: int, frames: int, start: int = 0):
-    """Algorithmic FP64 work per phase on a sample of the workload, counted by
-    the oracle's op-counting build (oracle/libopcount.so: every add, subtract,
-    multiply, divide and square root the restatement executes)."""
+def op_counts(scenes_mod, world, start_state: dict, n_scenes: int, frames: int):
+    """Algorithmic FP64 work per phase on a sample of the timed window (the
+    first n_scenes scenes from the window-start state, `frames` frames),
+    counted by the oracle's op-counting build (oracle/libopcount.so: every
+    add, subtract, multiply, divide and square root the restatement
+    executes)."""
     sys.path.insert(0, str(ROOT / "tests"))
     from _libs import SimWorld, _bind  # measurement infrastructure only
 
@@ -91,10 +175,9 @@ def op_counts(scenes_mod, n_scenes: int, frames: int, start: int = 0):
         return None
     lib = _bind(ctypes.CDLL(str(path)), "orc_")
     lib.orc_opcount_get.argtypes = [ctypes.c_void_p]
-    world = scenes_mod.batched_uniform(n_scenes, first_seed=0)
-    sim = SimWorld("orc", world, lib=lib)
-    if start:
-        sim.step(start)  # the counted frames start where the timed window starts
+    sample = scenes_mod.batched_uniform(n_scenes, first_seed=0)
+    sim = SimWorld("orc", sample, lib=lib)
+    sim.set_state(_scene_slice(world, start_state, 0, n_scenes))
     lib.orc_opcount_reset()
     pairs = contacts = 0
     for _ in range(frames):
@@ -109,7 +192,7 @@ def op_counts(scenes_mod, n_scenes: int, frames: int, start: int = 0):
     d = {n: int(ops[i]) for i, n in enumerate(names)}
     d["solve"] = d["position"] + d["velocity_update"] + d["velocity_solve"]
     return {"ops": d, "scenes": n_scenes, "frames": frames, "narrow_pairs": int(pairs), "contacts": int(contacts),
-            "body_substeps": world.dynamic_bodies() * 20 * frames}
+            "body_substeps": sample.dynamic_bodies() * 20 * frames}
 
 
 TRAFFIC_FILE = ROOT / "profiles" / "traffic_window_f300_batched_uniform_4096x16.json"
@@ -173,16 +256,25 @@ def large_run(pkg, sc, n: int, frames: int):
     t0 = time.perf_counter()
     world = sc.large_scene(n)
     t_build = time.perf_counter() - t0
+    t0 = time.perf_counter()
     gw = pkg.World(world)
-    gw.step(2)
+    t_upload = time.perf_counter() - t0
+    init = gw.state()
+    gw.step(2)  # first launches and graph capture
+    gw.set_state(init)
     gw.synchronize()
     t0 = time.perf_counter()
     gw.step(frames, sync=False)
     gw.synchronize()
     el = time.perf_counter() - t0
-    out = {"bodies": n, "frames_timed": frames, "after_frames": 2,
+    st = gw.state()
+    out = {"workload": f"large single scene: {n} dynamic bodies (60% cubes, 40% ico42, scale U(0.5,1.5)) above a "
+                       f"static floor, frames 0-{frames - 1}, dt=1/60, 20 substeps; uniform-grid broad phase, "
+                       "graph-coloured solve",
+           "bodies": n, "frames": frames,
            "value": n * 20 * frames / el, "unit": "body-substeps/s", "ms_per_frame": 1e3 * el / frames,
-           "scene_build_s": t_build, "error_word": int(gw.errors()[0])}
+           "scene_build_s": t_build, "upload_s": t_upload, "error_word": int(gw.errors()[0]),
+           "lowest_body_y": float(st["position"][1:, 1].min()), "active_bodies": int(st["active"][1:].sum())}
     gw.close()
     log("large scene: " + json.dumps(out))
     return out
@@ -240,18 +332,22 @@ def copy_calibration(nbytes: int):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=600, help="timed frames of the whole batch")
+    ap.add_argument("--steps", type=int, default=60, help="timed frames of the whole batch")
     ap.add_argument("--warmup", type=int, default=5, help="untimed frames before the timed region")
+    ap.add_argument("--start-frame", type=int, default=300,
+                    help="frame the timed window starts at: the batch is stepped there untimed first "
+                         "(frame 300 of the 600-frame run: a mix of falling, resting and sleeping scenes whose "
+                         "per-frame cost matches the run's mean)")
+    ap.add_argument("--run-frames", type=int, default=600, help="frames of the whole run timed from frame 0")
     ap.add_argument("--scenes", type=int, default=4096)
     ap.add_argument("--plain-scenes", type=int, default=16)
     ap.add_argument("--plain-frames", type=int, default=60)
-    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
+    ap.add_argument("--cpu-budget", type=float, default=8.0, help="seconds of CPU baseline work per leg")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel timing run")
     ap.add_argument("--op-scenes", type=int, default=8, help="scenes in the op-count sample")
-    ap.add_argument("--op-start", type=int, default=0,
-                    help="frame the op-count sample starts at (the frame a --load-state window starts at)")
+    ap.add_argument("--ops-out", metavar="JSON", help="write the window's op counts (per phase and per launch)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tune-from", type=int, default=90, help="frame the autotuner's trials start from")
     ap.add_argument("--tune-frames", type=int, default=6)
@@ -270,21 +366,32 @@ def main():
     ap.add_argument("--no-uneven", action="store_true", help="skip the batched uneven measurement")
     ap.add_argument("--uneven-scenes", type=int, default=1024)
     ap.add_argument("--uneven-frames", type=int, default=600)
-    ap.add_argument("--large-bodies", type=int, default=0, help="also time one large scene of this many bodies")
-    ap.add_argument("--large-frames", type=int, default=30)
-    ap.add_argument("--save-state", metavar="NPZ", help="after the timed frames, save the batch's body state")
-    ap.add_argument("--load-state", metavar="NPZ",
-                    help="start the timed frames from a saved body state (a window deep in the run, for profiles)")
+    ap.add_argument("--large-bodies", type=int, default=100000,
+                    help="bodies of the large single scene (0: skip it)")
+    ap.add_argument("--large-frames", type=int, default=300)
+    ap.add_argument("--window-only", action="store_true",
+                    help="only the batched window (profiling runs): no whole run, no other workloads")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run one device-to-device copy of MIB MiB (a kernel of known traffic that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
+    # one process of the all-cores CPU leg (started by cpu_baseline)
+    ap.add_argument("--cpu-worker", metavar="NPZ", help=argparse.SUPPRESS)
+    ap.add_argument("--worker-kind", help=argparse.SUPPRESS)
+    ap.add_argument("--worker-first", type=int, help=argparse.SUPPRESS)
+    ap.add_argument("--worker-scenes", type=int, help=argparse.SUPPRESS)
     args = ap.parse_args()
 
+    if args.cpu_worker:
+        cpu_worker(args)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on exactly one GPU: scenes are independent, so a multi-GPU run is the same "
               "kernel on each GPU's share of the scenes; no figure at several GPUs is produced.",
               file=sys.stderr)
         sys.exit(2)
+    if args.window_only:
+        args.no_adjoint = args.no_uneven = args.no_plain = args.no_cpu = True
+        args.large_bodies = 0
 
     pkg = _package()
     sc = pkg.scenes
@@ -298,10 +405,6 @@ def main():
     t0 = time.perf_counter()
     gw = pkg.World(world)
     log(f"uploaded world in {time.perf_counter() - t0:.1f} s")
-    if args.load_state:
-        with np.load(args.load_state, allow_pickle=False) as z:
-            gw.set_state({k: z[k] for k in z.files})
-        log(f"start state loaded from {args.load_state}")
     init = gw.state()
 
     # schedule: the package's autotuner times each narrow-phase variant on a
@@ -326,31 +429,52 @@ def main():
             f"manifold {gw.manifold_group} lanes/pair, "
             f"solve {gw.solve_lanes} lanes/scene")
 
-    # warmup (first launches, code object load), then restore the initial state
-    if args.warmup > 0:
-        gw.step(args.warmup)
-        gw.set_state(init)
+    # the whole run from frame 0 (graph replay), which also reaches the
+    # window's start state
+    dyn = world.dynamic_bodies()
+    run = None
+    gw.step(2)
+    gw.set_state(init)
     gw.synchronize()
+    if not args.window_only and args.run_frames >= args.start_frame:
+        t0 = time.perf_counter()
+        gw.step(args.start_frame, dt=dt, substeps=substeps, sync=False)
+        gw.synchronize()
+        t_head = time.perf_counter() - t0
+        start_state = gw.state()
+        t0 = time.perf_counter()
+        gw.step(args.run_frames - args.start_frame, dt=dt, substeps=substeps, sync=False)
+        gw.synchronize()
+        el = t_head + time.perf_counter() - t0
+        run = {"frames": args.run_frames, "value": dyn * substeps * args.run_frames / el,
+               "ms_per_frame": 1e3 * el / args.run_frames, "error_scenes": int((gw.errors() != 0).sum())}
+        log(f"whole run: {args.run_frames} frames in {el:.3f} s -> {run['value']:.4g} body-substeps/s")
+    else:
+        gw.step(args.start_frame, dt=dt, substeps=substeps)
+        start_state = gw.state()
 
+    # the timed window: frames start_frame .. start_frame + steps - 1
+    if args.warmup > 0:
+        gw.set_state(start_state)
+        gw.step(args.warmup)
+    gw.set_state(start_state)
+    gw.synchronize()
     t0 = time.perf_counter()
     gw.step(args.steps, dt=dt, substeps=substeps, sync=False)
     gw.synchronize()
     elapsed = time.perf_counter() - t0
-    dyn = world.dynamic_bodies()
     value = dyn * substeps * args.steps / elapsed
-    if args.save_state:
-        np.savez(args.save_state, **gw.state())
-        log(f"state after the timed frames saved to {args.save_state}")
     errs = gw.errors()
-    narrow_group, manifold_group, solve_lanes = gw.narrow_group, gw.manifold_group, gw.solve_lanes
-    log(f"batched: {args.steps} frames in {elapsed:.3f} s -> {value:.4g} body-substeps/s; "
-        f"error words nonzero in {int((errs != 0).sum())} scenes; EPA non-convergence {gw.epa_failures()}")
+    narrow_group, manifold_group, solve_lanes, views = gw.narrow_group, gw.manifold_group, gw.solve_lanes, gw.views
+    log(f"batched window: frames {args.start_frame}-{args.start_frame + args.steps - 1} in {elapsed:.3f} s -> "
+        f"{value:.4g} body-substeps/s; error words nonzero in {int((errs != 0).sum())} scenes; "
+        f"EPA non-convergence {gw.epa_failures()}")
 
-    # Per-kernel durations: the same run again from the initial state, with HIP
-    # events recorded around every launch on the world's stream.
+    # Per-kernel durations: the same window again, with HIP events recorded
+    # around every launch on the world's stream (eager launches, one view).
     ktimes = None
     if not args.no_profile:
-        gw.set_state(init)
+        gw.set_state(start_state)
         gw.set_profiling(True)
         gw.step(args.steps, dt=dt, substeps=substeps)
         ktimes = gw.kernel_times()
@@ -364,7 +488,6 @@ def main():
 
     plain = None
     if not args.no_plain:
-        ps = sc.batched_uniform(args.plain_scenes)
         worlds = []
         for s in range(args.plain_scenes):
             w1 = sc.World(name="plain")
@@ -372,26 +495,28 @@ def main():
             worlds.append(pkg.World(w1))
         for w1 in worlds:  # warm
             w1.step(1)
+        for s, w1 in enumerate(worlds):  # each scene at the window's start state
+            w1.set_state(_scene_slice(world, start_state, s, 1))
         t0 = time.perf_counter()
         for w1 in worlds:
             for _ in range(args.plain_frames):
                 w1.step(1, dt=dt, substeps=substeps)
                 w1.state()  # write-back to the host, as the drop-in does every call
         pt = time.perf_counter() - t0
-        plain = ps.dynamic_bodies() * substeps * args.plain_frames / pt
+        plain = 16 * args.plain_scenes * substeps * args.plain_frames / pt
         log(f"plain: {args.plain_scenes} scenes x {args.plain_frames} frames one scene per call -> {plain:.4g}")
         for w1 in worlds:
             w1.close()
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(sc, args.steps, args.cpu_budget)
+        cpu = cpu_baseline(sc, world, start_state, args.start_frame, args.steps, args.cpu_budget)
         log(f"cpu: {cpu}")
 
     roofline = None
     kernels = None
     if ktimes is not None:
-        oc = op_counts(sc, min(args.op_scenes, args.scenes), args.steps, args.op_start)
+        oc = op_counts(sc, world, start_state, min(args.op_scenes, args.scenes), args.steps)
         if oc is not None:
             scale = args.scenes / oc["scenes"]
             kernels = {}
@@ -411,7 +536,7 @@ def main():
                 ms, n = ktimes[cls]
                 flops = ops[cls] * scale
                 kernels[cls] = {"ms_total": ms, "launches": n, "avg_us": 1e3 * ms / max(n, 1),
-                                "algorithmic_flops": flops,
+                                "algorithmic_flops": flops, "flops_per_launch": flops / max(n, 1),
                                 "tflops": flops / (ms * 1e-3) / 1e12 if ms > 0 else None}
             # the dominant kernel: largest share of the frame
             names = {"narrow": ("k_gjk_epa (GJK + EPA + contact manifold per live pair)" if fused
@@ -430,13 +555,24 @@ def main():
                         "frac": k["tflops"] / FP64_PEAK_TFLOPS if k["tflops"] else None,
                         "traffic": traffic["bytes_per_launch"] if traffic else None,
                         "traffic_source": traffic["source"] if traffic else None,
-                        "flops_per_launch": k["algorithmic_flops"] / max(k["launches"], 1),
+                        "flops_per_launch": k["flops_per_launch"],
                         "avg_launch_us": k["avg_us"],
-                        "op_sample": f"{oc['scenes']} scenes x {oc['frames']} frames counted by the oracle's "
-                                     f"op-counting build, scaled x{scale:g} to {args.scenes} scenes; "
+                        "op_sample": f"{oc['scenes']} scenes x frames {args.start_frame}-"
+                                     f"{args.start_frame + args.steps - 1} counted by the oracle's op-counting "
+                                     f"build, scaled x{scale:g} to {args.scenes} scenes; "
                                      f"{unit_ops[0] / max(unit_ops[1], 1):.0f} FP64 ops per {unit_ops[2]}"}
             log("roofline: " + json.dumps(roofline))
+            if args.ops_out:
+                Path(args.ops_out).write_text(json.dumps(
+                    {"window": f"batched uniform {args.scenes} scenes, frames {args.start_frame}-"
+                               f"{args.start_frame + args.steps - 1} from the frame-{args.start_frame} state",
+                     "sample": {k2: oc[k2] for k2 in ("scenes", "frames", "narrow_pairs", "contacts",
+                                                      "body_substeps")},
+                     "scale": scale, "ops_sample": oc["ops"], "kernels": kernels,
+                     "schedule": {"narrow": narrow_group, "manifold": manifold_group, "solve": solve_lanes}},
+                    indent=1))
 
+    window = f"frames {args.start_frame}-{args.start_frame + args.steps - 1}"
     out = {
         "metric": METRIC,
         "value": value,
@@ -450,12 +586,16 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded scenes)",
-        "config": {"workload": "batched uniform: 4096 scenes x (floor + 8 cubes + 8 ico42), 600 frames, "
-                               "dt=1/60, 20 substeps, 1 position iteration",
-                   "scenes": args.scenes, "dynamic_bodies": dyn, "frames": args.steps, "substeps": substeps,
+        "config": {"workload": f"batched uniform: {args.scenes} scenes x (floor + 8 cubes + 8 ico42), {window} of "
+                               f"the {max(args.run_frames, args.start_frame + args.steps)}-frame run (stepped "
+                               f"untimed to frame {args.start_frame} first), dt=1/60, 20 substeps, "
+                               "1 position iteration",
+                   "scenes": args.scenes, "dynamic_bodies": dyn, "start_frame": args.start_frame,
+                   "frames": args.steps, "substeps": substeps,
                    "narrow_lanes_per_pair": narrow_group, "manifold_lanes_per_pair": manifold_group,
-                   "solve_lanes_per_scene": solve_lanes, "autotune_ms": tune},
-        "plain": {"value": plain, "unit": "body-substeps/s",
+                   "solve_lanes_per_scene": solve_lanes, "views": views, "autotune_ms": tune},
+        "whole_run": run,
+        "plain": {"value": plain, "unit": "body-substeps/s", "window": window,
                   "ratio_batched_over_plain": (value / plain) if plain else None},
         "adjoint": adjoint,
         "large_scene": large,

@@ -543,10 +543,15 @@ def large_scene(n: int, seed: int = 0, large_scene_bodies: int = 0, height: floa
     seeded uniformly in a (2L) x height x (2L) volume above a static floor
     (cube.obj x (L + 10, 1, L + 10) at y = -2), L = 100 sqrt(n / 10^5) so the
     density matches 10^5 bodies in 200 x 40 x 200. Positions are rejection-
-    sampled so that no two bounding spheres overlap. Mass = hull volume
+    sampled so that no two bounding spheres overlap: candidates are drawn in
+    order and a candidate is kept if its bounding sphere overlaps no kept
+    one (greedy in draw order, checked in batches against a k-d tree of the
+    kept bodies and pairwise within the batch). Mass = hull volume
     (density 1); friction 0.5/0.3, restitution 0. `ico_subdiv` = 2 gives
-    162-vertex icospheres instead."""
-    rng = Rng(0x5EED0000 + seed)
+    162-vertex icospheres instead. Draws come from numpy's PCG64 seeded with
+    0x5EED0000 + seed (candidates) and + 1 (orientations)."""
+    from scipy.spatial import cKDTree
+
     L = 100.0 * math.sqrt(n / 1.0e5)
     w = World(name=f"large_{n}", large_scene_bodies=large_scene_bodies)
     cube = w.shape(cube_corners())
@@ -555,32 +560,75 @@ def large_scene(n: int, seed: int = 0, large_scene_bodies: int = 0, height: floa
                fixed=True, material=(0.5, 0.3, 0.0))
     unit_volume = {cube: hull_volume(w.shapes[cube], (1.0, 1.0, 1.0)),
                    ico: hull_volume(w.shapes[ico], (1.0, 1.0, 1.0))}
-    cell = 2.0 * math.sqrt(3.0) * 1.5
-    grid: dict = {}
-    placed = 0
-    while placed < n:
-        is_cube = rng.uniform(0.0, 1.0) < 0.6
-        level = min(LARGE_SCALES - 1, int(rng.uniform(0.0, LARGE_SCALES)))
+    cand = np.random.Generator(np.random.PCG64(0x5EED0000 + seed))
+    orient = np.random.Generator(np.random.PCG64(0x5EED0000 + seed + 1))
+    max_bound = math.sqrt(3.0) * 1.5
+    # kept bodies: a k-d tree over the older ones (rebuilt every `rebuild`
+    # bodies) and a small one over the newer ones
+    kept_pos, kept_bound, kept_cube, kept_level = [], [], [], []
+    old_n, old_tree, old_pos, old_bound = 0, None, None, None
+    total, batch, rebuild = 0, 4096, 16384
+
+    def reject(pos, bound, ok, tree, tpos, tbound):
+        # a nearest neighbour closer than bound + 0.5 (the smallest bound)
+        # overlaps; the rest are checked against every body within reach
+        d, _ = tree.query(pos, k=1)
+        ok &= ~(d < bound + 0.5)
+        live = np.nonzero(ok)[0]
+        if not len(live):
+            return
+        m = cKDTree(pos[live]).sparse_distance_matrix(tree, 2.0 * max_bound, output_type="ndarray")
+        i, j = live[m["i"]], m["j"]
+        hit = ((tpos[j] - pos[i]) ** 2).sum(axis=1) < (tbound[j] + bound[i]) ** 2
+        ok[i[hit]] = False
+
+    while total < n:
+        u = cand.random((batch, 5))
+        is_cube = u[:, 0] < 0.6
+        level = np.minimum(LARGE_SCALES - 1, (u[:, 1] * LARGE_SCALES).astype(np.int64))
         r = 0.5 + level / (LARGE_SCALES - 1)
-        bound = math.sqrt(3.0) * r if is_cube else r
-        x = rng.uniform(-L, L)
-        y = -1.0 + bound + 0.5 + rng.uniform(0.0, height)
-        z = rng.uniform(-L, L)
-        key = (int(math.floor(x / cell)), int(math.floor(y / cell)), int(math.floor(z / cell)))
-        ok = True
-        for dx in (-1, 0, 1):
-            for dy in (-1, 0, 1):
-                for dz in (-1, 0, 1):
-                    for (px, py, pz, pb) in grid.get((key[0] + dx, key[1] + dy, key[2] + dz), ()):
-                        if (px - x) ** 2 + (py - y) ** 2 + (pz - z) ** 2 < (pb + bound) ** 2:
-                            ok = False
-        if not ok:
-            continue
-        grid.setdefault(key, []).append((x, y, z, bound))
-        shape = cube if is_cube else ico
-        w.add_body(shape, (r, r, r), (x, y, z), rng.quaternion(), mass=unit_volume[shape] * r * r * r,
+        bound = np.where(is_cube, math.sqrt(3.0) * r, r)
+        pos = np.stack([-L + 2.0 * L * u[:, 2], -1.0 + bound + 0.5 + height * u[:, 3], -L + 2.0 * L * u[:, 4]],
+                       axis=1)
+        ok = np.ones(batch, dtype=bool)
+        if old_tree is not None:
+            reject(pos, bound, ok, old_tree, old_pos, old_bound)
+        if total > old_n:
+            new_pos = np.concatenate(kept_pos)[old_n:]
+            new_bound = np.concatenate(kept_bound)[old_n:]
+            reject(pos, bound, ok, cKDTree(new_pos), new_pos, new_bound)
+        idx = np.nonzero(ok)[0]
+        if len(idx) > 1:  # greedy in draw order among the batch's survivors
+            pairs = cKDTree(pos[idx]).query_pairs(2.0 * max_bound, output_type="ndarray")
+            if len(pairs):
+                a, b = idx[pairs[:, 0]], idx[pairs[:, 1]]
+                lo, hi = np.minimum(a, b), np.maximum(a, b)
+                hit = ((pos[lo] - pos[hi]) ** 2).sum(axis=1) < (bound[lo] + bound[hi]) ** 2
+                lo, hi = lo[hit], hi[hit]
+                order = np.argsort(lo, kind="stable")
+                for p, q in zip(lo[order].tolist(), hi[order].tolist()):
+                    if ok[p]:
+                        ok[q] = False
+        idx = np.nonzero(ok)[0][: n - total]
+        kept_pos.append(pos[idx])
+        kept_bound.append(bound[idx])
+        kept_cube.append(is_cube[idx])
+        kept_level.append(level[idx])
+        total += len(idx)
+        if total - old_n >= rebuild:
+            old_pos, old_bound = np.concatenate(kept_pos), np.concatenate(kept_bound)
+            old_n, old_tree = total, cKDTree(old_pos)
+    kept_pos, kept_cube, kept_level = (np.concatenate(kept_pos), np.concatenate(kept_cube),
+                                       np.concatenate(kept_level))
+    q = orient.random((n, 3))
+    a, b = np.sqrt(1.0 - q[:, 0]), np.sqrt(q[:, 0])
+    quat = np.stack([a * np.sin(2 * math.pi * q[:, 1]), a * np.cos(2 * math.pi * q[:, 1]),
+                     b * np.sin(2 * math.pi * q[:, 2]), b * np.cos(2 * math.pi * q[:, 2])], axis=1)
+    for i in range(n):
+        shape = cube if kept_cube[i] else ico
+        r = 0.5 + int(kept_level[i]) / (LARGE_SCALES - 1)
+        w.add_body(shape, (r, r, r), tuple(kept_pos[i]), tuple(quat[i]), mass=unit_volume[shape] * r * r * r,
                    material=(0.5, 0.3, 0.0))
-        placed += 1
     w.end_scene()
     return w
 

@@ -20,6 +20,8 @@ ROOT = Path(__file__).resolve().parent.parent
 PKG_DIR = ROOT / "raw-physics_amd"
 ORACLE_SO = ROOT / "oracle" / "liboracle.so"
 REF_SO = ROOT / "oracle" / "_ref" / "libref.so"
+# the reference at its release flags (-O2 -ffast-math): CPU-baseline timing only
+REF_FAST_SO = ROOT / "oracle" / "_ref" / "libref_fast.so"
 
 
 def load_package():
@@ -78,10 +80,10 @@ def load(kind: str):
         lib = pkg.library()  # raises if librawphys.so is missing: no fallback
         _cache[kind] = lib
         return lib
-    path = {"orc": ORACLE_SO, "ref": REF_SO}[kind]
+    path = {"orc": ORACLE_SO, "ref": REF_SO, "ref_fast": REF_FAST_SO}[kind]
     if not path.exists():
         return None
-    lib = _bind(ctypes.CDLL(str(path)), kind + "_")
+    lib = _bind(ctypes.CDLL(str(path)), kind.split("_")[0] + "_")
     _cache[kind] = lib
     return lib
 
@@ -92,7 +94,7 @@ class SimWorld:
     def __init__(self, kind: str, world: scenes.World, lib=None):
         self.kind = kind
         self.lib = lib or load(kind)
-        self.p = kind + "_"
+        self.p = kind.split("_")[0] + "_"
         self.world = world
         self.desc, self.keep = world.desc()
         self.h = self._f("world_create")(ctypes.byref(self.desc))
