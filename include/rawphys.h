@@ -362,6 +362,15 @@ void rp_world_body_info(rp_world* w, double* inverse_mass, double* inertia, doub
                         double* bounding_radius);
 uint32_t rp_world_broad_pairs(rp_world* w, uint32_t scene, uint32_t* pairs, uint32_t cap);
 uint32_t rp_world_contacts(rp_world* w, uint32_t scene, uint32_t i, uint32_t j, double* out, uint32_t cap);
+/* Solve classes of a large scene's items in the last frame run: item t is
+ * the scene's external constraint t, then (with collisions on) its live pair
+ * t - (external constraints) in pair order; class 0..23 is a colour (items
+ * of one colour share no non-fixed body and are solved in parallel), 24 the
+ * class solved last, in item order. Position and velocity passes take the
+ * classes in ascending order. Returns the item count (0 for a scene that is
+ * not large) and writes min(count, cap) classes. No counterpart in the
+ * reference, whose solve is sequential (pbd.cpp:615-620, 646-711). */
+uint32_t rp_world_item_classes(rp_world* w, uint32_t scene, int32_t* classes, uint32_t cap);
 void rp_world_hull(rp_world* w, uint32_t body, uint32_t counts[6], double* vertices, double* face_normals,
                    uint32_t* face_off, uint32_t* face_el, uint32_t* vf_off, uint32_t* vf, uint32_t* vn_off,
                    uint32_t* vn, uint32_t* ff_off, uint32_t* ff);

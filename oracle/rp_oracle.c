@@ -325,6 +325,13 @@ struct orc_world {
     logcontactarr log_contacts;
     uint32_t log_pair_cur, log_contact_cur;
     int log_error;
+    /* Solve order given for one frame of one scene (orc_world_set_solve_order):
+     * the class of every item of that frame, items solved by ascending class
+     * and in item order within a class. */
+    int64_t order_scene;          /* -1: none */
+    int32_t* order_class;
+    uint32_t order_n;
+    uint32_t order_status;
 };
 
 /* ------------------------------------------------------------------ */
@@ -1414,18 +1421,30 @@ static constraint contact_to_constraint(const body* e1, const body* e2, uint32_t
 /* ------------------------------------------------------------------ */
 typedef ARR(constraint) constraintarr;
 
-/* broad_get_collision_pairs (broad.cpp:6-29) */
+/* broad_get_collision_pairs (broad.cpp:6-29). A pair whose x offset alone
+ * exceeds the bound is skipped before the test: sqrt(fl(x*x)) == |x| for
+ * correctly rounded IEEE arithmetic and the other two terms only add, so
+ * the reference's length is at least |x| and its test fails too. The skip
+ * is plain double arithmetic (not counted as algorithmic work). */
 static uint32_t broad(const scene* sc, uint32_t** out) {
     u32arr pairs = {0};
+    double* xr = malloc(sizeof(double) * 2 * (sc->nb ? sc->nb : 1));
+    for (uint32_t i = 0; i < sc->nb; ++i) {
+        xr[2 * i] = RD(sc->b[i].x.x);
+        xr[2 * i + 1] = RD(sc->b[i].radius);
+    }
     for (uint32_t i = 0; i < sc->nb; ++i) {
         const body* e1 = &sc->b[i];
+        double x1 = xr[2 * i], r1 = xr[2 * i + 1];
         for (uint32_t j = i + 1; j < sc->nb; ++j) {
+            if (fabs(x1 - xr[2 * j]) > (r1 + xr[2 * j + 1]) + 0.1) continue;
             const body* e2 = &sc->b[j];
             real dd = len(sub(e1->x, e2->x));
             real maxd = e1->radius + e2->radius + 0.1;
             if (dd <= maxd) { ARR_PUSH(pairs, i); ARR_PUSH(pairs, j); }
         }
     }
+    free(xr);
     *out = pairs.d;
     return pairs.n / 2;
 }
@@ -1475,6 +1494,76 @@ static void islands_and_sleep(scene* sc, const uint32_t* pairs, uint32_t np, rea
     free(parent);
 }
 
+/* A frame's solve items in a given class order (orc_world_set_solve_order):
+ * item t < nc_ext is external constraint t, item nc_ext + k the k-th pair the
+ * narrow phase visits (all of its contacts, in order). perm lists the items
+ * by ascending class, in item order within a class. The frame's items must
+ * match the given count, no class may hold two items that share a non-fixed
+ * body (such items would not commute), and classes must lie in [0, 64). */
+typedef struct {
+    int on;
+    uint32_t n;
+    uint32_t* perm;
+    uint32_t* first; /* [n + 1] constraint range of each item in the substep's list */
+} solve_order;
+
+static void order_begin(orc_world* w, scene* sc, const uint32_t* pairs, uint32_t np, int32_t collisions,
+                        solve_order* so) {
+    memset(so, 0, sizeof(*so));
+    if (w->order_scene != (int64_t)(sc - w->s)) return;
+    w->order_scene = -1;
+    uint32_t n = sc->nc_ext;
+    uint32_t* ib = malloc(sizeof(uint32_t) * 2 * (sc->nc_ext + np + 1));
+    for (uint32_t k = 0; k < sc->nc_ext; ++k) { ib[2 * k] = sc->ext[k].b1; ib[2 * k + 1] = sc->ext[k].b2; }
+    for (uint32_t k = 0; collisions && k < np; ++k) {
+        const body *e1 = &sc->b[pairs[2 * k]], *e2 = &sc->b[pairs[2 * k + 1]];
+        if ((e1->fixed || !e1->active) && (e2->fixed || !e2->active)) continue;
+        ib[2 * n] = pairs[2 * k];
+        ib[2 * n + 1] = pairs[2 * k + 1];
+        ++n;
+    }
+    if (n != w->order_n) {
+        w->order_status |= 1u;
+        free(ib);
+        return;
+    }
+    for (uint32_t t = 0; t < n; ++t)
+        if (w->order_class[t] < 0 || w->order_class[t] >= 64) {
+            w->order_status |= 4u;
+            free(ib);
+            return;
+        }
+    /* counting sort by class, stable */
+    uint32_t cnt[65] = {0};
+    for (uint32_t t = 0; t < n; ++t) ++cnt[w->order_class[t] + 1];
+    for (int c = 0; c < 64; ++c) cnt[c + 1] += cnt[c];
+    so->perm = malloc(sizeof(uint32_t) * (n ? n : 1));
+    for (uint32_t t = 0; t < n; ++t) so->perm[cnt[w->order_class[t]]++] = t;
+    /* no two items of one class share a non-fixed body */
+    int32_t* seen = malloc(sizeof(int32_t) * (sc->nb ? sc->nb : 1));
+    for (uint32_t i = 0; i < sc->nb; ++i) seen[i] = -1;
+    for (uint32_t k = 0; k < n; ++k) {
+        uint32_t t = so->perm[k];
+        int32_t c = w->order_class[t];
+        for (int side = 0; side < 2; ++side) {
+            uint32_t b = ib[2 * t + side];
+            if (sc->b[b].fixed || (side == 1 && b == ib[2 * t])) continue;
+            if (seen[b] == c) w->order_status |= 2u;
+            seen[b] = c;
+        }
+    }
+    free(seen);
+    free(ib);
+    so->n = n;
+    so->first = malloc(sizeof(uint32_t) * (n + 1));
+    so->on = 1;
+}
+
+static void order_end(solve_order* so) {
+    free(so->perm);
+    free(so->first);
+}
+
 /* pbd_simulate_with_constraints (pbd.cpp:468-747) for one scene */
 static void simulate(orc_world* w, scene* sc, real dt, uint32_t substeps, uint32_t iters, int32_t collisions) {
     if (dt <= 0.0) return;
@@ -1485,6 +1574,8 @@ static void simulate(orc_world* w, scene* sc, real dt, uint32_t substeps, uint32
     islands_and_sleep(sc, pairs, np, dt);
     constraintarr cons = {0};
     contactarr cts = {0};
+    solve_order so;
+    order_begin(w, sc, pairs, np, collisions, &so);
     for (uint32_t s = 0; s < substeps; ++s) {
         /* integrate (pbd.cpp:537-577) */
         ORC_PHASE(PH_INTEGRATE);
@@ -1514,8 +1605,10 @@ static void simulate(orc_world* w, scene* sc, real dt, uint32_t substeps, uint32
             constraint c = sc->ext[k];
             c.lambda = c.lambda2 = c.lambda3 = 0.0;
             c.lambda_t = c.lambda_n = 0.0;
+            if (so.on) so.first[k] = k;
             ARR_PUSH(cons, c);
         }
+        uint32_t visited = sc->nc_ext;
         if (collisions) {
             /* Transform each hull in a visited pair once for this substep:
              * the reference transforms both bodies of each pair right before
@@ -1540,6 +1633,8 @@ static void simulate(orc_world* w, scene* sc, real dt, uint32_t substeps, uint32
                     sc->err |= ORC_ERR_ACTIVITY;
                 if ((e1->fixed || !e1->active) && (e2->fixed || !e2->active)) continue;
                 ++w->last_narrow_pairs;
+                if (so.on) so.first[visited] = cons.n;
+                ++visited;
                 if (w->log_mode == 2) {  /* replay the logged constraints of this visit */
                     if (w->log_pair_cur >= w->log_pairs.n || w->log_pairs.d[w->log_pair_cur].i1 != i1 ||
                         w->log_pairs.d[w->log_pair_cur].i2 != i2) {
@@ -1580,10 +1675,19 @@ static void simulate(orc_world* w, scene* sc, real dt, uint32_t substeps, uint32
                 w->last_contacts += cts.n;
             }
         }
+        if (so.on) so.first[so.n] = cons.n;
         /* position solve (pbd.cpp:615-620) */
         ORC_PHASE(PH_POSITION);
-        for (uint32_t it = 0; it < iters; ++it)
-            for (uint32_t k = 0; k < cons.n; ++k) solve_constraint(sc, &cons.d[k], h);
+        for (uint32_t it = 0; it < iters; ++it) {
+            if (!so.on) {
+                for (uint32_t k = 0; k < cons.n; ++k) solve_constraint(sc, &cons.d[k], h);
+                continue;
+            }
+            for (uint32_t m = 0; m < so.n; ++m) {
+                uint32_t t = so.perm[m];
+                for (uint32_t k = so.first[t]; k < so.first[t + 1]; ++k) solve_constraint(sc, &cons.d[k], h);
+            }
+        }
         /* velocity update (pbd.cpp:623-643) */
         ORC_PHASE(PH_VELOCITY_UPDATE);
         for (uint32_t j = 0; j < sc->nb; ++j) {
@@ -1600,11 +1704,15 @@ static void simulate(orc_world* w, scene* sc, real dt, uint32_t substeps, uint32
         }
         /* velocity solve (pbd.cpp:646-711) */
         ORC_PHASE(PH_VELOCITY_SOLVE);
-        for (uint32_t k = 0; k < cons.n; ++k) {
-            constraint* c = &cons.d[k];
-            if (c->type == ORC_COLLISION_CONSTRAINT) velocity_solve(c, &sc->b[c->b1], &sc->b[c->b2], h);
+        for (uint32_t m = 0; m < (so.on ? so.n : 1); ++m) {
+            uint32_t k0 = so.on ? so.first[so.perm[m]] : 0, k1 = so.on ? so.first[so.perm[m] + 1] : cons.n;
+            for (uint32_t k = k0; k < k1; ++k) {
+                constraint* c = &cons.d[k];
+                if (c->type == ORC_COLLISION_CONSTRAINT) velocity_solve(c, &sc->b[c->b1], &sc->b[c->b2], h);
+            }
         }
     }
+    order_end(&so);
     ARR_FREE(cons);
     ARR_FREE(cts);
     free(pairs);
@@ -1624,6 +1732,7 @@ static void st4(double* p, uint32_t i, qt q) { p[4 * i] = RD(q.x); p[4 * i + 1] 
 orc_world* orc_world_create(const orc_world_desc* d) {
     ORC_PHASE(PH_SETUP);
     orc_world* w = calloc(1, sizeof(orc_world));
+    w->order_scene = -1;
     w->ns = d->num_scenes;
     w->s = calloc(w->ns ? w->ns : 1, sizeof(scene));
     w->nb_total = d->scene_body_offset[d->num_scenes];
@@ -1773,6 +1882,16 @@ void orc_world_contact_log(orc_world* w, int mode) {
 
 int orc_world_contact_log_error(const orc_world* w) { return w->log_error; }
 
+void orc_world_set_solve_order(orc_world* w, uint32_t scene_idx, const int32_t* item_class, uint32_t n) {
+    free(w->order_class);
+    w->order_class = malloc(sizeof(int32_t) * (n ? n : 1));
+    if (n) memcpy(w->order_class, item_class, sizeof(int32_t) * n);
+    w->order_n = n;
+    w->order_scene = scene_idx < w->ns ? (int64_t)scene_idx : -1;
+}
+
+uint32_t orc_world_solve_order_status(const orc_world* w) { return w->order_status; }
+
 /* Copies src's contact log into dst (for replay in a world built with other
  * parameters). */
 void orc_world_contact_log_copy(orc_world* dst, const orc_world* src) {
@@ -1786,6 +1905,7 @@ void orc_world_destroy(orc_world* w) {
     if (!w) return;
     free(w->log_pairs.d);
     free(w->log_contacts.d);
+    free(w->order_class);
     for (uint32_t si = 0; si < w->ns; ++si) {
         for (uint32_t k = 0; k < w->s[si].nb; ++k) {
             body* e = &w->s[si].b[k];

@@ -118,6 +118,17 @@ void orc_world_destroy(orc_world* w);
 void orc_world_contact_log(orc_world* w, int mode);
 int orc_world_contact_log_error(const orc_world* w);
 void orc_world_contact_log_copy(orc_world* dst, const orc_world* src);
+/* Solve order for the next frame of one scene (a check of a parallel solve
+ * that groups the frame's items into classes): item t < external
+ * constraints is external constraint t, item (external constraints) + k the
+ * k-th body pair the narrow phase visits, with all of its contacts. The
+ * frame solves the items by ascending class and in item order within a
+ * class, in both position iterations and the velocity solve; otherwise the
+ * step is unchanged. Status bits (cumulative): 1 the frame's item count
+ * differs from n, 2 two items of one class share a non-fixed body, 4 a
+ * class outside [0, 64); with bit 1 or 4 the frame runs in list order. */
+void orc_world_set_solve_order(orc_world* w, uint32_t scene, const int32_t* item_class, uint32_t n);
+uint32_t orc_world_solve_order_status(const orc_world* w);
 uint32_t orc_world_num_bodies(const orc_world* w);
 
 /* Runs `frames` frames of the reference's caller pattern on every scene

@@ -49,6 +49,7 @@ def library():
         "rp_world_body_info": (None, [_P, _P, _P, _P, _P]),
         "rp_world_broad_pairs": (u32, [_P, u32, _P, u32]),
         "rp_world_contacts": (u32, [_P, u32, u32, u32, _P, u32]),
+        "rp_world_item_classes": (u32, [_P, u32, _P, u32]),
         "rp_world_hull": (None, [_P, u32, _P] + [_P] * 10),
         "rp_last_error": (u32, []),
         "rp_last_error_string": (ctypes.c_char_p, []),
@@ -188,6 +189,14 @@ class World:
 
     def adjoint_mismatches(self) -> int:
         return int(self.lib.rp_world_adjoint_mismatches(self.h))
+
+    def item_classes(self, scene: int) -> np.ndarray:
+        """Solve classes of a large scene's items in the last frame (colours
+        0..23, 24 = the serial class; empty for a scene that is not large)."""
+        n = int(self.lib.rp_world_item_classes(self.h, scene, None, 0))
+        out = np.zeros(max(n, 1), dtype=np.int32)
+        self.lib.rp_world_item_classes(self.h, scene, out.ctypes.data, n)
+        return out[:n]
 
     def gjk_cycle_exits(self) -> int:
         return int(self.lib.rp_world_gjk_cycle_exits(self.h))

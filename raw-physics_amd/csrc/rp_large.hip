@@ -481,15 +481,25 @@ __global__ void k_big_csr_fill(DevWorld w, BigScene b, int collisions) {
     }
 }
 
-// One Jones-Plassmann round: an uncoloured item whose priority beats every
-// uncoloured neighbour takes the smallest colour no coloured neighbour has.
-// Two neighbours never colour in the same round, so reads are stable.
-__global__ void k_big_colour_round(DevWorld w, BigScene b, int collisions) {
-    if (__hip_atomic_load(b.remaining, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
+// One Jones-Plassmann round over the previous round's colours `src`
+// (-1 = uncoloured), written with this round's to `dst`: an uncoloured item
+// whose priority beats every uncoloured neighbour takes the smallest colour
+// no coloured neighbour has. Reads see only the previous round, so the
+// colouring does not depend on thread timing, and two neighbours never
+// colour in the same round (one of them has the lower priority).
+__global__ void k_big_colour_round(DevWorld w, BigScene b, int collisions, const int* __restrict__ src,
+                                   int* __restrict__ dst) {
     int K = min(big_items(w, b, collisions), b.item_cap);
-    int* colour = w.item_level + b.item_base;
+    // all coloured: copy. (The count falls during this launch too, but it
+    // reaches 0 only once every item is coloured in src or by a block that
+    // has run, so a block that sees 0 late copies what it would have kept.)
+    bool done = __hip_atomic_load(b.remaining, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < K; t += gridDim.x * blockDim.x) {
-        if (colour[t] >= 0) continue;
+        int ct = src[t];
+        if (done || ct >= 0) {
+            dst[t] = ct;
+            continue;
+        }
         uint32_t pt = item_priority(t);
         int2 bb = item_bodies(w, b, t);
         bool best = true;
@@ -500,7 +510,7 @@ __global__ void k_big_colour_round(DevWorld w, BigScene b, int collisions) {
             for (int m = b.bi_start[body]; m < b.bi_start[body + 1]; ++m) {
                 int u = b.bi_items[m];
                 if (u == t) continue;
-                int cu = __hip_atomic_load(colour + u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                int cu = src[u];
                 if (cu < 0) {
                     uint32_t pu = item_priority(u);
                     if (pu > pt || (pu == pt && u > t)) { best = false; break; }
@@ -509,10 +519,13 @@ __global__ void k_big_colour_round(DevWorld w, BigScene b, int collisions) {
                 }
             }
         }
-        if (!best) continue;
+        if (!best) {
+            dst[t] = -1;
+            continue;
+        }
         int c = __ffsll((unsigned long long)~used) - 1;
         if (c < 0 || c >= BIG_COLOURS) c = BIG_SERIAL;
-        __hip_atomic_store(colour + t, c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        dst[t] = c;
         atomicSub(b.remaining, 1);
     }
 }
@@ -596,8 +609,14 @@ void big_colour(const DevWorld& w, const BigScene& b, bool collisions, hipStream
     scan(b.bi_cursor, b.bi_start, b.n, b.scan_blocks, st);
     hipLaunchKernelGGL(k_copy, dim3(grid_for(b.n, 256)), dim3(256), 0, st, b.bi_start, b.bi_cursor, b.n);
     hipLaunchKernelGGL(k_big_csr_fill, dim3(gi), dim3(256), 0, st, w, b, co);
+    // rounds alternate between item_level and item_order (free until
+    // k_big_order fills it); an even number of rounds ends in item_level
+    static_assert(BIG_COLOUR_ROUNDS % 2 == 0, "the last colour round must write item_level");
+    int* colour = w.item_level + b.item_base;
+    int* spare = w.item_order + b.item_base;
     for (int r = 0; r < BIG_COLOUR_ROUNDS; ++r)
-        hipLaunchKernelGGL(k_big_colour_round, dim3(gi), dim3(256), 0, st, w, b, co);
+        hipLaunchKernelGGL(k_big_colour_round, dim3(gi), dim3(256), 0, st, w, b, co, (r & 1) ? spare : colour,
+                           (r & 1) ? colour : spare);
     hipLaunchKernelGGL(k_big_classes, dim3(1), dim3(1024), 0, st, w, b, co);
     hipLaunchKernelGGL(k_big_cursor, dim3(1), dim3(64), 0, st, w, b);
     hipLaunchKernelGGL(k_big_serial_flags, dim3(gi), dim3(256), 0, st, w, b, co);

@@ -766,6 +766,7 @@ struct rp_world {
     Recorder rec = {nullptr, nullptr, nullptr, 0};
     double* abar = nullptr;
     double* gparam = nullptr;
+    bool last_collisions = true;  // of the last frame run (its solve items include live pairs)
 };
 
 static void adjoint_free(rp_world* w) {
@@ -969,6 +970,7 @@ static hipError_t chunk_mark(rp_world* w, uint32_t f, int view, hipStream_t st) 
 }
 
 static uint32_t world_run(rp_world* w, const FrameParams& fp, uint32_t frames, bool sync) {
+    w->last_collisions = fp.collisions;
     if (w->use_graphs && !w->timer.on && frames > 0) {
         hipError_t e = frame_graph(w, fp);
         // the views start after the main stream's earlier work and the main
@@ -1184,6 +1186,22 @@ uint32_t rp_world_broad_pairs(rp_world* w, uint32_t scene, uint32_t* out, uint32
         out[2 * k + 1] = (uint32_t)pr[k].y;
     }
     return (uint32_t)n;
+}
+
+uint32_t rp_world_item_classes(rp_world* w, uint32_t scene, int32_t* out, uint32_t cap) {
+    const DevWorld& d = w->im.d;
+    const BigScene* bs = nullptr;
+    for (const BigScene& b : w->im.big)
+        if (b.s == (int)scene) bs = &b;
+    if (!bs || (int)scene >= d.ns) return 0;
+    if (hipStreamSynchronize(w->stream) != hipSuccess) return 0;
+    int32_t live = 0;
+    (void)hipMemcpy(&live, d.live_count + scene, 4, hipMemcpyDeviceToHost);
+    int64_t next = w->hw.scene_ext_off[scene + 1] - w->hw.scene_ext_off[scene];
+    int64_t K = std::min<int64_t>(next + (w->last_collisions ? live : 0), bs->item_cap);
+    if (out && K > 0)
+        (void)hipMemcpy(out, d.item_level + bs->item_base, std::min<int64_t>(K, cap) * 4, hipMemcpyDeviceToHost);
+    return (uint32_t)K;
 }
 
 uint32_t rp_world_contacts(rp_world* w, uint32_t scene, uint32_t i, uint32_t j, double* out, uint32_t cap) {

@@ -10,11 +10,21 @@ graph-coloured solve (rp_large.hip).
   path must be deterministic (bit-identical reruns).
 * The same scene through the per-scene path (threshold above the body count)
   stays bit-exact with the oracle.
+* The coloured solve itself is pinned bit for bit: the oracle, told the
+  device's solve classes of each frame's items, solves the items class by
+  class (orc_world_set_solve_order) and must give the device's bits; it also
+  checks that no class holds two items sharing a non-fixed body.
+* At the large-scene workload's full size (10^5 bodies), frames 0-1: grid
+  pairs equal the oracle's all-pairs list, the first frame equals the
+  oracle in the device's class order, reruns give the same bits, nothing is
+  below the floor and no error word is set.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
-from _libs import SimWorld, scenes, state_bits_equal
+from _libs import SimWorld, load, scenes, state_bits_equal
 
 pytestmark = pytest.mark.gpu
 
@@ -117,3 +127,101 @@ def test_large_scene_per_scene_path_bit_exact():
         o.step(15)
         assert not state_bits_equal(g.state(), o.state())
     assert g.error(0) == 0
+
+
+def _classes(g: SimWorld, scene: int = 0) -> np.ndarray:
+    lib = g.lib
+    n = lib.rp_world_item_classes(g.h, scene, None, 0)
+    out = np.zeros(max(n, 1), dtype=np.int32)
+    lib.rp_world_item_classes(g.h, scene, out.ctypes.data, n)
+    return out[:n]
+
+
+def _order_api(o: SimWorld):
+    lib = o.lib
+    lib.orc_world_set_solve_order.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32]
+    lib.orc_world_solve_order_status.restype = ctypes.c_uint32
+    lib.orc_world_solve_order_status.argtypes = [ctypes.c_void_p]
+    return lib
+
+
+def _frame_in_class_order(g: SimWorld, o: SimWorld):
+    """One frame on the device, then the same frame on the oracle in the
+    device's class order; returns the class counts."""
+    lib = _order_api(o)
+    g.step(1)
+    cls = _classes(g)
+    assert len(cls) > 0
+    lib.orc_world_set_solve_order(o.h, 0, cls.ctypes.data, len(cls))
+    o.step(1)
+    assert lib.orc_world_solve_order_status(o.h) == 0
+    return np.bincount(cls, minlength=25)
+
+
+def test_coloured_solve_bit_exact_in_class_order():
+    w = _large(True)
+    g = SimWorld("rp", w)
+    o = SimWorld("orc", w)
+    g.step(35)  # the pile is landing: contacts on the floor and between bodies
+    o.set_state(g.state())
+    counts = np.zeros(25, dtype=np.int64)
+    for f in range(8):
+        counts += _frame_in_class_order(g, o)
+        bad = state_bits_equal(g.state(), o.state())
+        assert not bad, f"frame {36 + f}: {bad}"
+    print("items per class over 8 frames:", counts.tolist())
+    assert (counts[1:24] > 0).sum() >= 2  # several colours were used
+    assert g.error(0) == 0 and o.error(0) == 0
+
+
+def test_class_order_check_catches_a_conflict():
+    """The oracle flags a class that holds two items sharing a body (here:
+    every item in class 0)."""
+    w = _large(True)
+    g = SimWorld("rp", w)
+    o = SimWorld("orc", w)
+    g.step(35)
+    o.set_state(g.state())
+    g.step(1)
+    cls = np.zeros_like(_classes(g))
+    lib = _order_api(o)
+    lib.orc_world_set_solve_order(o.h, 0, cls.ctypes.data, len(cls))
+    o.step(1)
+    assert lib.orc_world_solve_order_status(o.h) & 2
+
+
+LARGE_N = 100_000
+
+
+def test_large_scene_full_size_frames_0_1():
+    """The large-scene workload at full size: 10^5 bodies above the floor."""
+    w = scenes.large_scene(LARGE_N)
+    assert w.num_bodies == LARGE_N + 1
+    g = SimWorld("rp", w)
+    o = SimWorld("orc", w)
+    init = g.state()
+    cap = 1 << 22
+    for f in range(2):
+        # grid pairs at the device's state equal the oracle's all-pairs test
+        o.set_state(g.state())
+        pg = np.zeros((cap, 2), dtype=np.uint32)
+        po = np.zeros((cap, 2), dtype=np.uint32)
+        ng = g._f("world_broad_pairs")(g.h, 0, pg.ctypes.data, cap)
+        no = o._f("world_broad_pairs")(o.h, 0, po.ctypes.data, cap)
+        assert ng == no and ng < cap, (f, ng, no)
+        assert np.array_equal(pg[:ng], po[:no]), f
+        if f == 0:
+            # the first frame in the device's class order, bit for bit
+            _frame_in_class_order(g, o)
+            bad = state_bits_equal(g.state(), o.state())
+            assert not bad, bad
+        else:
+            g.step(1)
+    st = g.state()
+    assert np.all(st["position"][1:, 1] > -1.0), "a body is below the floor's top face"
+    assert np.all(st["active"][1:] == 1)
+    assert g.error(0) == 0
+    # a rerun from the start gives the same bits
+    g.set_state(init)
+    g.step(2)
+    assert not state_bits_equal(g.state(), st)
