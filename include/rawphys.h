@@ -317,6 +317,12 @@ int32_t rp_world_narrow_group(const rp_world* w);
 int32_t rp_world_set_manifold_group(rp_world* w, int32_t lanes_per_pair);
 int32_t rp_world_manifold_group(const rp_world* w);
 
+/* Most blocks (one wave each) of a narrow-phase launch, 1..8192, or 0 for the
+ * default 8192. Lane groups take the frame's live pairs in grid-stride
+ * order, so fewer blocks give each group several pairs in turn; results are
+ * identical. Returns 0 or -1. */
+int32_t rp_world_set_narrow_block_cap(rp_world* w, int32_t blocks);
+
 /* Concurrent views: graph replay cuts the world into up to 4 scene ranges
  * of about equal body counts and replays each on a stream of its own, so
  * that one range's latency-bound kernels overlap another's (scenes never

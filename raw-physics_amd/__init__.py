@@ -65,6 +65,7 @@ def library():
         "rp_world_narrow_group": (i32, [_P]),
         "rp_world_set_manifold_group": (i32, [_P, i32]),
         "rp_world_manifold_group": (i32, [_P]),
+        "rp_world_set_narrow_block_cap": (i32, [_P, i32]),
         "rp_world_set_solve_lanes": (i32, [_P, i32]),
         "rp_world_set_colour_launches": (i32, [_P, i32]),
         "rp_world_solve_lanes": (i32, [_P]),

@@ -23,6 +23,7 @@ struct DevWorld {
     int narrow_group;               // lanes per pair in GJK + EPA: 1 (lane per pair) or 4, 8, 16
     int manifold_group;             // lanes per pair in the contact manifold: 1, 4, 8, 16
     int manifold_blocks;
+    int narrow_block_cap;           // most blocks of a narrow-phase launch (0: the default, 8192)
     int stage_cap;                  // LDS vertices per pair: 2 x the largest hull
     int small_nv;                   // > 0: pairs of two hulls of <= small_nv vertices go to their own list
     int stage_small;                // LDS vertices per pair of that list: 2 x small_nv

@@ -850,10 +850,10 @@ bool narrow_group_supported(const DevWorld& w, int g) {
 // grid-stride order; frames with more live pairs than 4096 blocks hold ran
 // 5 % faster with the larger cap (batched 4096-scene workload, 600 frames).
 int narrow_blocks_for(const DevWorld& w, int g, int64_t pcap) {
-    (void)w;
     int64_t per_block = 64 / g;
     int64_t blocks = (pcap + per_block - 1) / per_block;
-    return (int)std::max<int64_t>(1, std::min<int64_t>(blocks, 256 * 32));
+    int64_t cap = w.narrow_block_cap > 0 ? w.narrow_block_cap : 256 * 32;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(blocks, cap));
 }
 
 hipError_t launch_broad(const DevWorld& w, hipStream_t st, const BigScene* bigs, int nbig) {

@@ -1419,6 +1419,18 @@ int32_t rp_world_set_manifold_group(rp_world* w, int32_t lanes_per_pair) {
 
 int32_t rp_world_manifold_group(const rp_world* w) { return w->im.d.manifold_group; }
 
+int32_t rp_world_set_narrow_block_cap(rp_world* w, int32_t blocks) {
+    if (blocks < 0 || blocks > 256 * 32) return -1;
+    (void)hipStreamSynchronize(w->stream);
+    DevWorld& d = w->im.d;
+    d.narrow_block_cap = blocks;
+    d.narrow_blocks = narrow_blocks_for(d, d.narrow_group, w->im.pair_capacity);
+    d.manifold_blocks = narrow_blocks_for(d, d.manifold_group ? d.manifold_group : 1, w->im.pair_capacity);
+    if (!w->im.ensure_group_work()) return -1;
+    ++w->sched_gen;
+    return 0;
+}
+
 int32_t rp_world_set_solve_lanes(rp_world* w, int32_t lanes_per_scene) {
     if (lanes_per_scene != 16 && lanes_per_scene != 32 && lanes_per_scene != 64) return -1;
     (void)hipStreamSynchronize(w->stream);
@@ -1500,6 +1512,8 @@ struct DropIn {
 };
 static DropIn g_dropin;
 
+static Body entity_body(const rp_entity* e);
+
 static bool dropin_build(rp_entity** ents, size_t n, rp_constraint* cons, HostWorld& hw, uint32_t& err) {
     hw.scene_err.assign(1, 0);
     std::unordered_map<uint64_t, int32_t> id2idx;
@@ -1550,9 +1564,10 @@ static bool dropin_build(rp_entity** ents, size_t n, rp_constraint* cons, HostWo
         hw.body_col_off.push_back((int32_t)hw.cols.size());
         hw.body_hull.push_back(first_hull);
         hw.body_scale.push_back(to_v3(e->world_scale));
-        Body b;
-        memset(&b, 0, sizeof(b));
-        hw.body.push_back(b);
+        // the entity's record, so that what the image derives from it at
+        // build time (a large scene's large-body set and grid cell, from the
+        // bounding radii) matches what the step then reads
+        hw.body.push_back(entity_body(e));
     }
     hw.scene_body_off.push_back((int32_t)n);
     for (size_t k = 0; k < la_len(cons); ++k) {
@@ -1677,6 +1692,9 @@ extern "C" uint32_t rp_pbd_simulate(double dt, rp_entity** ents, rp_constraint* 
     for (size_t i = 0; i < n; ++i) {
         key.push_back(ents[i]);
         key.push_back(ents[i]->colliders);
+        uint64_t rbits;  // a large scene's grid is built from the radii
+        memcpy(&rbits, &ents[i]->bounding_sphere_radius, 8);
+        key.push_back((const void*)(uintptr_t)rbits);
         for (size_t ci = 0; ci < la_len(ents[i]->colliders); ++ci) {
             const rp_collider& c = ents[i]->colliders[ci];
             uint32_t rbits = 0;

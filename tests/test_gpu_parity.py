@@ -156,6 +156,50 @@ def test_uneven_scenes_match_oracle():
     assert exact >= 5
 
 
+# Scenes of the batched uneven workload itself (batched_uneven(1024) uses
+# seeds 0..1023): six with more than 128 bodies (up to 252), seven with joint
+# chains, every one with 8-, 42- and 162-vertex hulls.
+UNEVEN_SEEDS = [6, 16, 27, 36, 70, 107, 0, 3, 26, 33, 41, 59]
+# Joint limits call asin / sin / cos (pbd.cpp:177, quaternion.cpp:7-10); the
+# device libm may differ from glibc in the last bit, and those differences
+# propagate through the chain and whatever it touches. Positions and
+# rotations within JOINT_TOL after 120 frames; velocities are position
+# differences over h = 1/1200 (pbd.cpp:629-642), so within JOINT_TOL / h.
+JOINT_TOL = 1e-9
+
+
+def test_uneven_full_size_scenes_match_oracle():
+    w = scenes.World(name="uneven_full")
+    for sd in UNEVEN_SEEDS:
+        scenes.uneven_scene(w, sd)
+    assert max(np.diff(w.scene_body_offset)) > 250
+    o = SimWorld("orc", w)
+    g = SimWorld("rp", w)
+    off, con = w.scene_body_offset, w.scene_constraint_offset
+    for frame in (30, 60, 90, 120):
+        o.step(30)
+        g.step(30)
+        so, sg = o.state(), g.state()
+        worst = 0.0
+        for s in range(w.num_scenes):
+            sl = slice(off[s], off[s + 1])
+            part_o = {k: v[sl] for k, v in so.items()}
+            part_g = {k: v[sl] for k, v in sg.items()}
+            if con[s + 1] == con[s]:
+                bad = state_bits_equal(part_o, part_g)
+                assert not bad, f"scene seed {UNEVEN_SEEDS[s]}, frame {frame}: {bad}"
+                continue
+            for k, tol in (("position", JOINT_TOL), ("rotation", JOINT_TOL),
+                           ("linear_velocity", JOINT_TOL * 1200.0), ("angular_velocity", JOINT_TOL * 1200.0)):
+                d = np.abs(part_o[k] - part_g[k]).max()
+                worst = max(worst, d / tol)
+                assert d <= tol, f"scene seed {UNEVEN_SEEDS[s]}, frame {frame}, {k}: {d:.3e}"
+            assert np.array_equal(part_o["active"], part_g["active"])
+        print(f"frame {frame}: chain scenes within {worst:.3f} of their tolerance")
+    for s in range(w.num_scenes):
+        assert g.error(s) == 0 and o.error(s) == 0
+
+
 @pytest.mark.parametrize("lanes", [64, 32, 16])
 def test_every_solve_schedule_bit_exact(lanes):
     """The solve's schedules (1, 2 or 4 scenes per wave) give identical results,
@@ -228,6 +272,32 @@ def test_views_bit_exact(views):
         assert g.error(s) == 0
 
 
+@pytest.mark.parametrize("blocks", [1, 3])
+def test_few_narrow_blocks_bit_exact(blocks):
+    """With the narrow-phase grid capped at a few one-wave blocks, each lane
+    group takes many live pairs in turn (grid-stride), reusing its LDS slice
+    and working-memory slot: fused, split (GJK + EPA, then manifold) and
+    size-class launches against the oracle."""
+    uneven = scenes.batched_uneven(6, first_seed=41, max_bodies=48)
+    for world, frames, narrow, manifold in ((scenes.batched_uniform(6, first_seed=500), 60, 8, 0),
+                                            (scenes.batched_uniform(6, first_seed=500), 60, 8, 8),
+                                            (scenes.batched_uniform(6, first_seed=500), 60, 1, 1),
+                                            (uneven, 40, 8, 0)):
+        o = SimWorld("orc", world)
+        g = SimWorld("rp", world)
+        assert g.lib.rp_world_set_narrow_group(g.h, narrow) == 0
+        assert g.lib.rp_world_set_manifold_group(g.h, manifold) == 0
+        assert g.lib.rp_world_set_narrow_block_cap(g.h, blocks) == 0
+        for done in range(0, frames, 20):
+            o.step(20)
+            g.step(20)
+            bad = state_bits_equal(o.state(), g.state())
+            assert not bad, f"{world.name} narrow {narrow} manifold {manifold} blocks {blocks}, frame {done + 20}: {bad}"
+        for s in range(world.num_scenes):
+            assert g.error(s) == 0
+    assert g.lib.rp_world_set_narrow_block_cap(g.h, 8193) != 0
+
+
 def test_size_classes_change_nothing():
     """With hulls above 64 vertices, the fused schedule visits small-hull and
     large-hull pairs in two lists (LDS slices sized per list): the same bits
@@ -252,9 +322,19 @@ def test_size_classes_change_nothing():
             assert not bad, f"{world.name}, {lanes} lanes, {views} views: {bad}"
             for s in range(world.num_scenes):
                 assert a.error(s) == b.error(s)
+    # the large scene is solved class by class (coloured): the oracle solves
+    # it in the device's class order, the small scenes in list order
     o, g = SimWorld("orc", big), SimWorld("rp", big)
-    o.step(30)
-    g.step(30)
+    o.lib.orc_world_set_solve_order.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32]
+    o.lib.orc_world_solve_order_status.restype = ctypes.c_uint32
+    for f in range(30):
+        g.step(1)
+        n = g.lib.rp_world_item_classes(g.h, 0, None, 0)
+        cls = np.zeros(max(n, 1), dtype=np.int32)
+        g.lib.rp_world_item_classes(g.h, 0, cls.ctypes.data, n)
+        o.lib.orc_world_set_solve_order(o.h, 0, cls.ctypes.data, n)
+        o.step(1)
+    assert o.lib.orc_world_solve_order_status(ctypes.c_void_p(o.h)) == 0
     bad = state_bits_equal(o.state(), g.state())
     assert not bad, f"{big.name} against the oracle: {bad}"
 
