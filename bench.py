@@ -216,7 +216,7 @@ def committed_traffic(kernel_class: str, schedule: dict):
     return {"bytes_per_launch": k["bytes_per_launch"], "source": f"{TRAFFIC_FILE.relative_to(ROOT)}: {d['window']}"}
 
 
-def adjoint_run(pkg, sc, n_scenes: int, frames: int):
+def adjoint_run(pkg, sc, n_scenes: int, frames: int, check_scenes: int = 256):
     """Differentiable workload: n_scenes cubes thrown onto tilted static boxes,
     `frames` frames; the checkpointed forward pass, then the backward pass of
     L = sum |x_final - target|^2. Adjoint steps/s = scenes x frames x
@@ -246,7 +246,19 @@ def adjoint_run(pkg, sc, n_scenes: int, frames: int):
            "forward_checkpointed_s": t_fwd, "backward_s": t_bwd, "forward_error": int(err),
            "recompute_mismatches": gw.adjoint_mismatches(),
            "grad_norm_v0": float(np.linalg.norm(gs[dyn, 7:10])), "grad_norm_mass": float(np.linalg.norm(gp[dyn, 0]))}
+    amp = gw.adjoint_amplification()[dyn]
     gw.close()
+    # Gradient check against FP64 central differences (h = 1e-6, and 1e-7 for
+    # reproducibility) of perturbed runs on the device, on the first scenes
+    # of the workload at several horizons (raw_physics_amd.gradcheck).
+    from raw_physics_amd import gradcheck
+
+    out["amplification_valid"] = gradcheck.AMPLIFICATION_VALID
+    out["valid_scenes"] = int((amp <= gradcheck.AMPLIFICATION_VALID).sum())
+    out["gradient_check"] = {}
+    for horizon in sorted({30, 60, frames}):
+        r = gradcheck.gradient_check(pkg, min(check_scenes, n_scenes), 0, horizon)
+        out["gradient_check"][f"{horizon}_frames"] = r["summary"]
     log("adjoint: " + json.dumps(out))
     return out
 
@@ -263,10 +275,13 @@ def large_run(pkg, sc, n: int, frames: int):
     gw.step(2)  # first launches and graph capture
     gw.set_state(init)
     gw.synchronize()
-    t0 = time.perf_counter()
-    gw.step(frames, sync=False)
-    gw.synchronize()
-    el = time.perf_counter() - t0
+    el = 0.0
+    for done in range(0, frames, 30):  # chunks with a progress line each
+        k = min(30, frames - done)
+        t0 = time.perf_counter()
+        gw.step(k)
+        el += time.perf_counter() - t0
+        log(f"large scene: frames {done}-{done + k - 1} at {1e3 * el / (done + k):.1f} ms/frame so far")
     st = gw.state()
     out = {"workload": f"large single scene: {n} dynamic bodies (60% cubes, 40% ico42, scale U(0.5,1.5)) above a "
                        f"static floor, frames 0-{frames - 1}, dt=1/60, 20 substeps; uniform-grid broad phase, "

@@ -16,7 +16,16 @@
 #include <string.h>
 #include <vector>
 
+#ifdef RP_LINK_DROPIN
+/* Built against librawphys.so instead of pbd.o: the step's declarations keep
+ * default visibility so that the calls bind to the library's exports. */
+#pragma GCC visibility push(default)
 #include "physics/pbd.h"
+extern "C" void physics_simulate(Entity** entities, r64 dt);
+#pragma GCC visibility pop
+#else
+#include "physics/pbd.h"
+#endif
 #include "physics/collider.h"
 #include "physics/broad.h"
 #include "examples/examples_util.h"
@@ -173,6 +182,40 @@ API void ref_world_step(void* wp, double dt, uint32_t frames, uint32_t substeps,
             for (u32 i = 0; i < array_length(ents); ++i) entity_clear_forces(ents[i]);
         }
     }
+}
+
+/* The frame driver with a choice of entry point: 0 pbd_simulate_with_constraints
+ * (as ref_world_step), 1 pbd_simulate (pbd.h:93; scenes without external
+ * constraints), 2 physics_simulate(entities, dt) (20 substeps, 1 iteration,
+ * collisions on; built against librawphys.so only). Returns 0, or 1 for a
+ * mode this build or scene does not have. */
+API uint32_t ref_world_step_mode(void* wp, int32_t mode, double dt, uint32_t frames, uint32_t substeps, uint32_t iters,
+                                 int32_t collisions, double gravity) {
+    ref_world* w = (ref_world*)wp;
+#ifndef RP_LINK_DROPIN
+    if (mode == 2) return 1;
+#endif
+    if (mode < 0 || mode > 2) return 1;
+    for (size_t s = 0; s < w->scenes.size(); ++s)
+        if (mode != 0 && w->constraints[s] && array_length(w->constraints[s])) return 1;
+    for (uint32_t f = 0; f < frames; ++f) {
+        for (size_t s = 0; s < w->scenes.size(); ++s) {
+            Entity** ents = w->scenes[s];
+            for (u32 i = 0; i < array_length(ents); ++i)
+                colliders_update(ents[i]->colliders, ents[i]->world_position, &ents[i]->world_rotation);
+            const r64 GRAVITY = gravity;
+            for (u32 i = 0; i < array_length(ents); ++i)
+                entity_add_force(ents[i], (vec3){0.0, 0.0, 0.0}, (vec3){0.0, -GRAVITY * 1.0 / ents[i]->inverse_mass, 0.0},
+                                 false);
+            if (mode == 0) pbd_simulate_with_constraints(dt, ents, w->constraints[s], substeps, iters, collisions);
+            else if (mode == 1) pbd_simulate(dt, ents, substeps, iters, collisions);
+#ifdef RP_LINK_DROPIN
+            else physics_simulate(ents, dt);
+#endif
+            for (u32 i = 0; i < array_length(ents); ++i) entity_clear_forces(ents[i]);
+        }
+    }
+    return 0;
 }
 
 static Entity* body_at(ref_world* w, uint32_t g) {

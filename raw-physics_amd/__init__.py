@@ -61,6 +61,7 @@ def library():
         "rp_world_adjoint_forward": (u32, [_P, f64, u32, u32, u32, f64]),
         "rp_world_adjoint_backward": (u32, [_P, _P, _P, _P]),
         "rp_world_adjoint_mismatches": (u64, [_P]),
+        "rp_world_adjoint_amplification": (u32, [_P, _P]),
         "rp_world_set_narrow_group": (i32, [_P, i32]),
         "rp_world_narrow_group": (i32, [_P]),
         "rp_world_set_manifold_group": (i32, [_P, i32]),
@@ -188,12 +189,22 @@ class World:
             raise RuntimeError(f"adjoint_backward failed ({err:#x}): {last_error_string()}")
         return gs, gp
 
+    def adjoint_amplification(self) -> np.ndarray:
+        """Per body: the backward pass's largest state adjoint at a frame
+        start over the largest component of dl_dstate (gradient
+        amplification; see rawphys.h)."""
+        out = np.zeros(self.nb)
+        err = self.lib.rp_world_adjoint_amplification(self.h, out.ctypes.data)
+        if err:
+            raise RuntimeError(f"adjoint_amplification failed ({err:#x}): {last_error_string()}")
+        return out
+
     def adjoint_mismatches(self) -> int:
         return int(self.lib.rp_world_adjoint_mismatches(self.h))
 
     def item_classes(self, scene: int) -> np.ndarray:
         """Solve classes of a large scene's items in the last frame (colours
-        0..23, 24 = the serial class; empty for a scene that is not large)."""
+        0..62, 63 = the serial class; empty for a scene that is not large)."""
         n = int(self.lib.rp_world_item_classes(self.h, scene, None, 0))
         out = np.zeros(max(n, 1), dtype=np.int32)
         self.lib.rp_world_item_classes(self.h, scene, out.ctypes.data, n)

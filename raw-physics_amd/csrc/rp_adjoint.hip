@@ -121,7 +121,7 @@ __device__ void adj_substep(const DevWorld& w, const Recorder& r, int step, int 
 
 __global__ void __launch_bounds__(64) k_adjoint_frame(DevWorld w, Recorder r, const int32_t* scene_dyn,
                                                       const Body* end_state, double* abar, double* gparam,
-                                                      FrameParams fp) {
+                                                      double* amax, FrameParams fp) {
     constexpr int SPB = 64 / ADJ_LANES;
     int s = blockIdx.x * SPB + (int)threadIdx.x / ADJ_LANES;
     int k = (int)threadIdx.x % ADJ_LANES;
@@ -170,15 +170,20 @@ __global__ void __launch_bounds__(64) k_adjoint_frame(DevWorld w, Recorder r, co
         __syncthreads();
     }
     if (bd >= 0 && k < 13) abar[13 * bd + k] = a_out[slot][k];
+    if (bd >= 0 && k == 0) {  // largest state adjoint at a frame start so far
+        double m = amax[bd];
+        for (int j = 0; j < 13; ++j) m = fmax(m, fabs(a_out[slot][j]));
+        amax[bd] = m;
+    }
     if (mismatch) atomicAdd(&w.stats[STAT_ADJ_MISMATCH], 1ull);
     if (err && bd >= 0 && k == 0) atomicOr(&w.scene_err[s], err);
 }
 
 hipError_t launch_adjoint_frame(const DevWorld& w, const Recorder& rec, const int32_t* scene_dyn, const Body* end_state,
-                                double* abar, double* gparam, const FrameParams& fp, hipStream_t st) {
+                                double* abar, double* gparam, double* amax, const FrameParams& fp, hipStream_t st) {
     constexpr int SPB = 64 / ADJ_LANES;
     hipLaunchKernelGGL(k_adjoint_frame, dim3((w.ns + SPB - 1) / SPB), dim3(64), 0, st, w, rec, scene_dyn, end_state,
-                       abar, gparam, fp);
+                       abar, gparam, amax, fp);
     return hipGetLastError();
 }
 

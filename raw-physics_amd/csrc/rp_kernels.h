@@ -95,11 +95,15 @@ struct DevWorld {
 // A large scene: grid broad phase, parallel islands and a graph-coloured
 // solve instead of the per-scene wave kernels. Device buffers are owned by
 // the world.
-constexpr int BIG_COLOURS = 24;             // parallel colours; items beyond them go to one serial class
+// A settled pile of 10^5 bodies has about 4.9 items (body pairs) per body;
+// greedy colouring of its item conflict graph needs up to about 40 colours,
+// and 24 colours with 32 rounds left 60 000 of its 490 000 items to the
+// serial class (one lane: seconds per frame).
+constexpr int BIG_COLOURS = 63;             // parallel colours (one 64-bit mask with the serial class)
 constexpr int BIG_SERIAL = BIG_COLOURS;     // class solved by one lane in item order
 constexpr int BIG_CLASSES = BIG_COLOURS + 1;
-constexpr int BIG_COLOUR_ROUNDS = 32;       // Jones-Plassmann rounds per frame
-constexpr int BIG_LAUNCH_CLASSES = 6;       // default: colours launched one by one before the one-block tail
+constexpr int BIG_COLOUR_ROUNDS = 64;       // Jones-Plassmann rounds per frame (launches end at once when all are coloured)
+constexpr int BIG_LAUNCH_CLASSES = 16;      // default: colours launched one by one before the one-block tail
 constexpr int LARGE_CHUNK = 256;            // bodies per wave in a large body's pair scan
 struct BigScene {
     int s, b0, n;                // scene, first body, bodies
@@ -178,9 +182,10 @@ hipError_t launch_frame(const DevWorld& w, const FrameParams& fp, hipStream_t st
 // Adjoint of one recorded frame for scenes with one dynamic body
 // (rp_adjoint.hip): abar [nb][13] is the adjoint of (x, q, v, w) at the end
 // of the frame on entry and at its start on exit; gparam [nb][3] accumulates
-// d/d(mass, static friction, dynamic friction).
+// d/d(mass, static friction, dynamic friction); amax [nb] keeps the largest
+// |abar| component of each dynamic body seen at a frame start.
 hipError_t launch_adjoint_frame(const DevWorld& w, const Recorder& rec, const int32_t* scene_dyn, const Body* end_state,
-                                double* abar, double* gparam, const FrameParams& fp, hipStream_t st);
+                                double* abar, double* gparam, double* amax, const FrameParams& fp, hipStream_t st);
 // Large-scene stages (rp_large.hip), in frame order.
 void big_broad(const DevWorld& w, const BigScene& b, hipStream_t st);
 void big_islands(const DevWorld& w, const BigScene& b, double dt, hipStream_t st);

@@ -766,17 +766,19 @@ struct rp_world {
     Recorder rec = {nullptr, nullptr, nullptr, 0};
     double* abar = nullptr;
     double* gparam = nullptr;
+    double* amax = nullptr;                 // [nb] largest state adjoint at a frame start
+    std::vector<double> dl_norm;            // [nb] largest |dL/dstate| component at the run's end
     bool last_collisions = true;  // of the last frame run (its solve items include live pairs)
 };
 
 static void adjoint_free(rp_world* w) {
-    void* ps[] = {w->ckpt, w->scene_dyn, w->rec.body, w->rec.ncontact, w->rec.contacts, w->abar, w->gparam};
+    void* ps[] = {w->ckpt, w->scene_dyn, w->rec.body, w->rec.ncontact, w->rec.contacts, w->abar, w->gparam, w->amax};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     w->ckpt = nullptr;
     w->scene_dyn = nullptr;
     w->rec = {nullptr, nullptr, nullptr, 0};
-    w->abar = w->gparam = nullptr;
+    w->abar = w->gparam = w->amax = nullptr;
     w->ck_frames = 0;
 }
 
@@ -1354,12 +1356,17 @@ uint32_t rp_world_adjoint_backward(rp_world* w, const double* dl_dstate, double*
             hipMalloc(&w->rec.ncontact, fp.substeps * w->rec.pair_cap * sizeof(int32_t)) != hipSuccess ||
             hipMalloc(&w->rec.contacts, fp.substeps * w->rec.pair_cap * MAXC * sizeof(Contact)) != hipSuccess ||
             hipMalloc(&w->abar, nb * 13 * sizeof(double)) != hipSuccess ||
-            hipMalloc(&w->gparam, nb * 3 * sizeof(double)) != hipSuccess) {
+            hipMalloc(&w->gparam, nb * 3 * sizeof(double)) != hipSuccess ||
+            hipMalloc(&w->amax, nb * sizeof(double)) != hipSuccess) {
             set_msg("adjoint_backward: device allocation failed");
             return RP_ERR_RUNTIME;
         }
     }
+    w->dl_norm.assign(nb, 0.0);
+    for (size_t i = 0; i < nb; ++i)
+        for (int j = 0; j < 13; ++j) w->dl_norm[i] = std::max(w->dl_norm[i], std::fabs(dl_dstate[13 * i + j]));
     if (hipMemcpy(w->abar, dl_dstate, nb * 13 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(w->amax, w->dl_norm.data(), nb * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(w->gparam, 0, nb * 3 * sizeof(double)) != hipSuccess)
         return RP_ERR_RUNTIME;
     for (int f = (int)w->ck_frames - 1; f >= 0; --f) {
@@ -1367,7 +1374,7 @@ uint32_t rp_world_adjoint_backward(rp_world* w, const double* dl_dstate, double*
         if (e == hipSuccess)
             e = launch_frame(d, fp, w->stream, nullptr, w->im.big.data(), (int)w->im.big.size(), &w->rec);
         if (e == hipSuccess)
-            e = launch_adjoint_frame(d, w->rec, w->scene_dyn, w->ckpt + (f + 1) * nb, w->abar, w->gparam, fp,
+            e = launch_adjoint_frame(d, w->rec, w->scene_dyn, w->ckpt + (f + 1) * nb, w->abar, w->gparam, w->amax, fp,
                                      w->stream);
         if (e != hipSuccess) {
             set_msg(std::string("adjoint_backward: ") + hipGetErrorString(e));
@@ -1382,6 +1389,18 @@ uint32_t rp_world_adjoint_backward(rp_world* w, const double* dl_dstate, double*
     if ((grad_state && hipMemcpy(grad_state, w->abar, nb * 13 * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) ||
         (grad_param && hipMemcpy(grad_param, w->gparam, nb * 3 * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess))
         return RP_ERR_RUNTIME;
+    return 0;
+}
+
+uint32_t rp_world_adjoint_amplification(rp_world* w, double* out) {
+    size_t nb = w->hw.body.size();
+    if (!w->amax || w->dl_norm.size() != nb) {
+        set_msg("adjoint_amplification: no backward pass");
+        return RP_ERR_SELECT;
+    }
+    std::vector<double> m(nb);
+    if (hipMemcpy(m.data(), w->amax, nb * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) return RP_ERR_RUNTIME;
+    for (size_t i = 0; i < nb; ++i) out[i] = w->dl_norm[i] > 0.0 ? m[i] / w->dl_norm[i] : 0.0;
     return 0;
 }
 

@@ -532,6 +532,44 @@ def ramps(num_scenes: int, first_seed: int = 0):
     return w, targets
 
 
+# Differentiable-workload parameters of a ramp scene's cube, with the
+# adjoint's output for each: ("state", column of (x, q, v, w)) or
+# ("param", column of (mass, static friction, dynamic friction)).
+RAMP_PARAMS = {"vx": ("state", 7), "vy": ("state", 8), "vz": ("state", 9),
+               "wx": ("state", 10), "wy": ("state", 11), "wz": ("state", 12),
+               "mass": ("param", 0), "mu_s": ("param", 1), "mu_d": ("param", 2)}
+
+
+def _perturb(w: World, b: int, name: str, delta: float) -> None:
+    """Moves parameter `name` of body b (global index) by delta."""
+    if name[0] in "vw" and len(name) == 2:
+        arr = w.body_linear_velocity if name[0] == "v" else w.body_angular_velocity
+        t = list(arr[b])
+        t["xyz".index(name[1])] += delta
+        arr[b] = tuple(t)
+    elif name == "mass":
+        w.body_mass[b] += delta
+    else:
+        m = list(w.body_material[b])
+        m[0 if name == "mu_s" else 1] += delta
+        w.body_material[b] = tuple(m)
+
+
+def ramps_perturbed(num_scenes: int, first_seed: int, params, h: float):
+    """Central-difference copies of ramps(num_scenes, first_seed): copy
+    (s * len(params) + k) * 2 + j is scene s with parameter params[k] of its
+    cube moved by +h (j = 0) or -h (j = 1). Returns the world and the loss
+    target of every copy; the cube of copy c is body 2c + 1."""
+    w = World(name=f"ramps_fd_{num_scenes}x{len(params)}")
+    targets = []
+    for s in range(num_scenes):
+        for name in params:
+            for sign in (1.0, -1.0):
+                targets.append(ramp_scene(w, first_seed + s))
+                _perturb(w, len(w.body_shape) - 1, name, sign * h)
+    return w, np.stack(targets)
+
+
 LARGE_SCALES = 64  # scale levels of the large scene (bodies of one level share a hull)
 
 

@@ -22,6 +22,9 @@ ORACLE_SO = ROOT / "oracle" / "liboracle.so"
 REF_SO = ROOT / "oracle" / "_ref" / "libref.so"
 # the reference at its release flags (-O2 -ffast-math): CPU-baseline timing only
 REF_FAST_SO = ROOT / "oracle" / "_ref" / "libref_fast.so"
+# the reference's objects without pbd.o, linked against librawphys.so (its
+# calls to pbd_simulate* bind to the library: the link-level drop-in)
+REF_DROPIN_SO = ROOT / "oracle" / "_ref" / "libref_dropin.so"
 
 
 def load_package():
@@ -80,7 +83,7 @@ def load(kind: str):
         lib = pkg.library()  # raises if librawphys.so is missing: no fallback
         _cache[kind] = lib
         return lib
-    path = {"orc": ORACLE_SO, "ref": REF_SO, "ref_fast": REF_FAST_SO}[kind]
+    path = {"orc": ORACLE_SO, "ref": REF_SO, "ref_fast": REF_FAST_SO, "ref_dropin": REF_DROPIN_SO}[kind]
     if not path.exists():
         return None
     lib = _bind(ctypes.CDLL(str(path)), kind.split("_")[0] + "_")

@@ -170,3 +170,47 @@ def test_dropin_large_scene_equals_batched_world():
     print(f"drop-in (coloured) vs reference (sequential) after 2 frames: max |dx| = {dx:.3e}")
     assert dx <= 1e-2
     assert np.all(sb["position"][1:, 1] > -1.0)
+
+
+def _step_mode(sim, mode, frames, **kw):
+    lib = sim.lib
+    lib.ref_world_step_mode.restype = ctypes.c_uint32
+    lib.ref_world_step_mode.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, ctypes.c_uint32,
+                                        ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32, ctypes.c_double]
+    coll = 1 if kw.get("collisions", True) else 0
+    assert lib.ref_world_step_mode(sim.h, mode, 1.0 / 60.0, frames, kw.get("substeps", 20), kw.get("iters", 1),
+                                   coll, 10.0) == 0
+
+
+@pytest.mark.skipif(load("ref_dropin") is None, reason="link-level drop-in harness (oracle/_ref) not built")
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_link_level_dropin_bit_exact(mode):
+    """The reference's own objects without pbd.o, linked against
+    librawphys.so (oracle/_ref/libref_dropin.so): the harness's calls to the
+    mangled pbd_simulate_with_constraints (mode 0) and pbd_simulate (1), and
+    to physics_simulate (2), run the library's step on entities the
+    reference's code built, through the reference's frame driver
+    (stack.cpp:86-104). Bit for bit against the reference's own step."""
+    for world, frames in ((scenes.stack_scene(seed=6), 90), (scenes.batched_uniform(2, first_seed=77), 90)):
+        a = SimWorld("ref", world)
+        b = SimWorld("ref_dropin", world)
+        _step_mode(a, 0 if mode == 0 else 1, frames)
+        _step_mode(b, mode, frames)
+        bad = state_bits_equal(a.state(), b.state())
+        assert not bad, f"{world.name}, mode {mode}: {bad}"
+    assert load("rp").rp_last_error() == 0
+
+
+@pytest.mark.skipif(load("ref_dropin") is None, reason="link-level drop-in harness (oracle/_ref) not built")
+def test_link_level_dropin_joint_chain():
+    """External constraints through the linked pbd_simulate_with_constraints,
+    built with the library's exported constraint constructors
+    (pbd.cpp:18-79); joint limits use the device libm (tolerance)."""
+    world = scenes.chains(1, length=6, first_seed=4)
+    a = SimWorld("ref", world)
+    b = SimWorld("ref_dropin", world)
+    _step_mode(a, 0, 90, collisions=False)
+    _step_mode(b, 0, 90, collisions=False)
+    sa, sb = a.state(), b.state()
+    assert np.allclose(sa["position"], sb["position"], rtol=0, atol=1e-9)
+    assert np.allclose(sa["rotation"], sb["rotation"], rtol=0, atol=1e-9)

@@ -66,13 +66,13 @@ def test_large_scene_is_deterministic():
 
 
 def test_colour_launch_schedules_give_identical_bits():
-    """Colour classes one launch each (25), the default split (6 launches,
+    """Colour classes one launch each (64), the default split (16 launches,
     then one block for the small classes and the serial class), and every
     class on one block (0): items of a colour are independent, so all three
     give the same bits."""
     w = _large(True)
     runs = []
-    for classes in (25, 6, 0):
+    for classes in (64, 16, 0):
         g = SimWorld("rp", w)
         assert g.lib.rp_world_set_colour_launches(g.h, classes) == 0
         g.step(45)
@@ -80,7 +80,7 @@ def test_colour_launch_schedules_give_identical_bits():
         assert g.error(0) == 0
     assert not state_bits_equal(runs[0], runs[1])
     assert not state_bits_equal(runs[0], runs[2])
-    assert g.lib.rp_world_set_colour_launches(g.h, 26) != 0
+    assert g.lib.rp_world_set_colour_launches(g.h, 65) != 0
 
 
 def test_large_scene_one_frame_within_tolerance():
@@ -155,7 +155,7 @@ def _frame_in_class_order(g: SimWorld, o: SimWorld):
     lib.orc_world_set_solve_order(o.h, 0, cls.ctypes.data, len(cls))
     o.step(1)
     assert lib.orc_world_solve_order_status(o.h) == 0
-    return np.bincount(cls, minlength=25)
+    return np.bincount(cls, minlength=64)
 
 
 def test_coloured_solve_bit_exact_in_class_order():
@@ -164,13 +164,13 @@ def test_coloured_solve_bit_exact_in_class_order():
     o = SimWorld("orc", w)
     g.step(35)  # the pile is landing: contacts on the floor and between bodies
     o.set_state(g.state())
-    counts = np.zeros(25, dtype=np.int64)
+    counts = np.zeros(64, dtype=np.int64)
     for f in range(8):
         counts += _frame_in_class_order(g, o)
         bad = state_bits_equal(g.state(), o.state())
         assert not bad, f"frame {36 + f}: {bad}"
     print("items per class over 8 frames:", counts.tolist())
-    assert (counts[1:24] > 0).sum() >= 2  # several colours were used
+    assert (counts[1:63] > 0).sum() >= 2  # several colours were used
     assert g.error(0) == 0 and o.error(0) == 0
 
 
