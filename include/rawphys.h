@@ -259,6 +259,21 @@ uint32_t rp_world_step(rp_world* w, double dt, uint32_t frames, uint32_t num_sub
 uint32_t rp_world_step_async(rp_world* w, double dt, uint32_t frames, uint32_t num_substeps, uint32_t num_pos_iters,
                              int32_t enable_collisions, double gravity);
 uint32_t rp_world_synchronize(rp_world* w);
+/* rp_world_step without allocating: no hipMalloc, hipFree, graph capture or
+ * other allocating call, for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid allocation). No
+ * counterpart in the reference. Same results as rp_world_step. The kernels
+ * are launched one by one on the world's stream (no graph replay, one view)
+ * and the narrow phase's lane-group working memory comes from a static
+ * device array of 64 MiB (4519 lane-group slots) instead of the world's own
+ * buffer; the narrow-phase grids are capped to the slots it holds. Every
+ * call on a device shares that device's array: calls that overlap on one
+ * device (worlds on different streams, other threads) race; calls on the
+ * library's one shared stream are ordered. The world must have been created
+ * on the current device. Blocks until done; returns 0 or RP_ERR_RUNTIME
+ * (rp_last_error_string says why). */
+uint32_t rp_world_step_noalloc(rp_world* w, double dt, uint32_t frames, uint32_t num_substeps, uint32_t num_pos_iters,
+                               int32_t enable_collisions, double gravity);
 
 void rp_world_get_state(rp_world* w, const rp_state* out);
 void rp_world_set_state(rp_world* w, const rp_state* in);

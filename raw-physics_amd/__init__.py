@@ -37,6 +37,7 @@ def library():
         "rp_world_step": (u32, [_P, f64, u32, u32, u32, i32, f64]),
         "rp_world_step_async": (u32, [_P, f64, u32, u32, u32, i32, f64]),
         "rp_world_synchronize": (u32, [_P]),
+        "rp_world_step_noalloc": (u32, [_P, f64, u32, u32, u32, i32, f64]),
         "rp_world_get_state": (None, [_P, ctypes.POINTER(scenes.StateC)]),
         "rp_world_set_state": (None, [_P, ctypes.POINTER(scenes.StateC)]),
         "rp_world_error": (u32, [_P, u32]),

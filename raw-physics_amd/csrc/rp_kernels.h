@@ -202,6 +202,12 @@ hipError_t launch_broad(const DevWorld& w, hipStream_t st, const BigScene* bigs 
 hipError_t launch_model_matrices(const DevWorld& w, void* out, bool f32, hipStream_t st);
 // Working-memory slots the grouped narrow-phase kernels of this schedule use.
 int64_t group_work_slots_needed(const DevWorld& w);
+// rp_world_step_noalloc's lane-group working memory: a static device array
+// of NOALLOC_SCRATCH_BYTES on each device (4519 slots of GROUP_WORK_BYTES).
+constexpr size_t NOALLOC_SCRATCH_BYTES = 64u << 20;
+// Its device address on the current device (looked up once per device), or
+// null if the lookup failed.
+unsigned char* noalloc_scratch();
 // Lanes per pair the narrow phase can use for this world (LDS permitting).
 bool narrow_group_supported(const DevWorld& w, int g);
 int narrow_blocks_for(const DevWorld& w, int g, int64_t pair_capacity);

@@ -21,6 +21,8 @@
 // one does not order two constraints. Every body therefore sees the same
 // sequence of updates as in the sequential loop and results are bit-identical.
 #include <algorithm>
+#include <mutex>
+#include <vector>
 
 #include "rp_device.h"
 
@@ -830,6 +832,28 @@ hipError_t launch_fp64_selftest(const double* a, const double* b, double* out, i
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_fp64_selftest, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, b, out, n);
     return hipGetLastError();
+}
+
+// Lane-group working memory of rp_world_step_noalloc: a file-scope device
+// array instead of a hipMalloc'd buffer. Each device that runs the library
+// holds its own copy until the process ends. Host code never passes the
+// array's name to a kernel (it is a host-side stand-in there): it takes the
+// device address from noalloc_scratch(), which looks it up once per device.
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+unsigned char* noalloc_scratch() {
+    static std::mutex mu;
+    static std::vector<unsigned char*> ptr;  // per device, null until looked up
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) return nullptr;
+    std::lock_guard<std::mutex> lock(mu);
+    if ((size_t)dev >= ptr.size()) ptr.resize(dev + 1, nullptr);
+    if (!ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+        ptr[dev] = (unsigned char*)p;
+    }
+    return ptr[dev];
 }
 
 int64_t group_work_slots_needed(const DevWorld& w) {

@@ -769,6 +769,7 @@ struct rp_world {
     double* amax = nullptr;                 // [nb] largest state adjoint at a frame start
     std::vector<double> dl_norm;            // [nb] largest |dL/dstate| component at the run's end
     bool last_collisions = true;  // of the last frame run (its solve items include live pairs)
+    int device = -1;              // HIP device the world was created on
 };
 
 static void adjoint_free(rp_world* w) {
@@ -924,6 +925,10 @@ static hipError_t frame_graph(rp_world* w, const FrameParams& fp) {
 }
 
 static bool world_init(rp_world* w) {
+    if (hipGetDevice(&w->device) != hipSuccess) {
+        set_msg("hipGetDevice failed");
+        return false;
+    }
     w->stream = shared_stream();
     if (!w->stream) {
         set_msg("hipStreamCreate failed");
@@ -1078,6 +1083,58 @@ uint32_t rp_world_step_async(rp_world* w, double dt, uint32_t frames, uint32_t s
                              double gravity) {
     return step_impl(w, dt, frames, substeps, iters, coll, gravity, false);
 }
+uint32_t rp_world_step_noalloc(rp_world* w, double dt, uint32_t frames, uint32_t substeps, uint32_t iters,
+                               int32_t coll, double gravity) {
+    g_last_err = 0;
+    if (!(dt > 0.0)) return 0;  // pbd.cpp:471
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev != w->device) {
+        set_msg("rp_world_step_noalloc: the world was created on another device");
+        return g_last_err = RP_ERR_RUNTIME;
+    }
+    unsigned char* ptr = noalloc_scratch();
+    if (!ptr) {
+        set_msg("rp_world_step_noalloc: hipGetSymbolAddress failed");
+        return g_last_err = RP_ERR_RUNTIME;
+    }
+    // the world's kernels with the lane-group working memory in the static
+    // array; the narrow-phase grids are capped to the groups it holds (lane
+    // groups take the live pairs in grid-stride order: identical results)
+    DevWorld d = w->im.d;
+    int64_t slots = (int64_t)(NOALLOC_SCRATCH_BYTES / GROUP_WORK_BYTES);
+    if (d.narrow_group > 1) d.narrow_blocks = (int)std::min<int64_t>(d.narrow_blocks, slots / (64 / d.narrow_group));
+    if (d.manifold_group > 1)
+        d.manifold_blocks = (int)std::min<int64_t>(d.manifold_blocks, slots / (64 / d.manifold_group));
+    if (d.narrow_blocks < 1 || d.manifold_blocks < 1 || group_work_slots_needed(d) > slots) {
+        set_msg("rp_world_step_noalloc: the schedule needs more lane-group working memory than the static array holds");
+        return g_last_err = RP_ERR_RUNTIME;
+    }
+    d.group_work = (char*)ptr;
+    d.group_work_slots = slots;
+    FrameParams fp;
+    fp.dt = dt;
+    fp.h = dt / substeps;
+    fp.gravity = gravity;
+    fp.substeps = substeps;
+    fp.iters = iters;
+    fp.collisions = coll != 0;
+    w->last_collisions = fp.collisions;
+    for (uint32_t f = 0; f < frames; ++f) {
+        hipError_t e = launch_frame(d, fp, w->stream, nullptr, w->im.big.data(), (int)w->im.big.size());
+        if (e == hipSuccess) e = chunk_mark(w, f, 0, w->stream);
+        if (e != hipSuccess) {
+            set_msg(std::string("kernel launch: ") + hipGetErrorString(e));
+            return g_last_err = RP_ERR_RUNTIME;
+        }
+    }
+    hipError_t e = hipStreamSynchronize(w->stream);
+    if (e != hipSuccess) {
+        set_msg(std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
+        return g_last_err = RP_ERR_RUNTIME;
+    }
+    return 0;
+}
+
 uint32_t rp_world_synchronize(rp_world* w) {
     hipError_t e = hipStreamSynchronize(w->stream);
     if (e != hipSuccess) { set_msg(hipGetErrorString(e)); return RP_ERR_RUNTIME; }
