@@ -386,6 +386,8 @@ def main():
     ap.add_argument("--large-frames", type=int, default=300)
     ap.add_argument("--window-only", action="store_true",
                     help="only the batched window (profiling runs): no whole run, no other workloads")
+    ap.add_argument("--only", choices=("uneven", "large", "adjoint"),
+                    help="run only this workload's leg (profiling runs) and print its result")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run one device-to-device copy of MIB MiB (a kernel of known traffic that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
@@ -414,6 +416,12 @@ def main():
 
     if args.copy_calibration:
         copy_calibration(args.copy_calibration << 20)
+    if args.only:
+        leg = {"uneven": lambda: uneven_run(pkg, sc, args.uneven_scenes, args.uneven_frames, args.tune_from),
+               "large": lambda: large_run(pkg, sc, args.large_bodies, args.large_frames),
+               "adjoint": lambda: adjoint_run(pkg, sc, args.adjoint_scenes, args.adjoint_frames)}[args.only]
+        print(json.dumps({"only": args.only, "result": leg()}), flush=True)
+        return
     t0 = time.perf_counter()
     world = sc.batched_uniform(args.scenes)
     log(f"built {args.scenes} scenes ({world.num_bodies} bodies) in {time.perf_counter() - t0:.1f} s")

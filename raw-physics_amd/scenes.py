@@ -489,6 +489,52 @@ def mixed_scene(w: World, seed: int, n_spheres: int = 5, n_cubes: int = 3, n_dum
     return w
 
 
+def spot_storm_scene(w: World, hulls, seed: int = 0, n: int = 2) -> World:
+    """The reference's spot_storm example (src/examples/spot_storm.cpp:56-137):
+    a static cube.obj x (50, 1, 50) floor at (0, -2, 0) (friction 0.5/0.5),
+    and n^3 bodies of the 11 convex pieces `hulls` (corner streams of
+    res/spot/spot-hull-1..11.obj) scaled by 2, mass 1, friction 0.8/0.8,
+    restitution 0, on its lattice (y from 2, x and z from -n, gap 3.5) with
+    random orientations (its generate_random_quaternion: axis U(0,1)^3,
+    angle U(-180, 180) degrees, from this seed instead of rand())."""
+    rng = Rng(0x5EED0000 + seed)
+    floor = w.shape(cube_corners())
+    w.add_body(floor, (50.0, 1.0, 50.0), (0.0, -2.0, 0.0), quaternion_new((0.0, 1.0, 0.0), 0.0), fixed=True,
+               material=(0.5, 0.5, 0.0))
+    pieces = tuple(w.shape(np.ascontiguousarray(h, dtype=np.float32)) for h in hulls)
+    gap, y = 3.5, 2.0
+    for i in range(n):
+        y += gap
+        x = -2.0 * (n / 2.0)
+        for j in range(n):
+            x += gap
+            z = -2.0 * (n / 2.0)
+            for k in range(n):
+                z += gap
+                axis = (rng.uniform(), rng.uniform(), rng.uniform())
+                angle = -180.0 + rng.uniform() * 360.0
+                w.add_body(pieces, (2.0, 2.0, 2.0), (x, y, z), quaternion_new(axis, angle), mass=1.0,
+                           material=(0.8, 0.8, 0.0))
+    w.end_scene()
+    return w
+
+
+def mirror_cube_scene(w: World, pieces) -> World:
+    """The reference's mirror_cube example (src/examples/mirror_cube.cpp:50-68):
+    a static cube.obj x (50, 1, 50) floor at (0, -2, 0) (friction 0.5/0.5)
+    and one body of the two convex pieces `pieces` (corner streams of
+    res/mirror_cube_collider1.obj and _collider2.obj), scale 1, at (0, 2, 0)
+    rotated 33 degrees about (1, 1, 1), mass 1, friction 0.8/0.8."""
+    floor = w.shape(cube_corners())
+    w.add_body(floor, (50.0, 1.0, 50.0), (0.0, -2.0, 0.0), quaternion_new((0.0, 1.0, 0.0), 0.0), fixed=True,
+               material=(0.5, 0.5, 0.0))
+    shapes = tuple(w.shape(np.ascontiguousarray(p, dtype=np.float32)) for p in pieces)
+    w.add_body(shapes, (1.0, 1.0, 1.0), (0.0, 2.0, 0.0), quaternion_new((1.0, 1.0, 1.0), 33.0), mass=1.0,
+               material=(0.8, 0.8, 0.0))
+    w.end_scene()
+    return w
+
+
 def batched_mixed(num_scenes: int, first_seed: int = 0, **kw) -> World:
     w = World(name=f"batched_mixed_{num_scenes}")
     for s in range(num_scenes):

@@ -399,3 +399,13 @@ def test_views_change_nothing():
         assert not bad, f"{world.name}: {bad}"
         for s in range(world.num_scenes):
             assert a.error(s) == b.error(s)
+
+
+def test_reference_asset_scenes_bit_exact():
+    """spot_storm and mirror_cube (tests/golden/assets.npz: the reference's
+    collider assets as its obj_parse returns them) on the batched device
+    path against the oracle, bit for bit, with the error words clear."""
+    from test_gpu_dropin import _asset_scenes
+
+    for world, frames in zip(_asset_scenes(), (90, 120)):
+        _pair(world, frames=frames, check_every=30)
