@@ -263,18 +263,46 @@ def adjoint_run(pkg, sc, n_scenes: int, frames: int, check_scenes: int = 256):
     return out
 
 
-def large_run(pkg, sc, n: int, frames: int):
-    """One large scene (grid broad phase, coloured solve): body-substeps/s."""
+def _leg_state(gw, load):
+    """The leg's start state: a saved one (profiling windows) or the world's own."""
+    if load:
+        with np.load(load, allow_pickle=False) as z:
+            gw.set_state({k: z[k] for k in z.files})
+        log(f"start state loaded from {load}")
+    return gw.state()
+
+
+def _leg_end(gw, save, profile):
+    """Saves the end state and returns the per-kernel HIP-event times (or None)."""
+    if save:
+        np.savez(save, **gw.state())
+        log(f"end state saved to {save}")
+    if not profile:
+        return None
+    kt = gw.kernel_times()
+    gw.set_profiling(False)
+    return kt
+
+
+def large_run(pkg, sc, n: int, frames: int, eager=False, load=None, save=None, profile=False):
+    """One large scene (grid broad phase, coloured solve): body-substeps/s.
+    load / save: start from / keep a saved state (profiling windows deep in
+    the run); eager: kernels launched one by one; profile: HIP events around
+    every launch (the frame loop then waits for each frame)."""
     t0 = time.perf_counter()
     world = sc.large_scene(n)
     t_build = time.perf_counter() - t0
     t0 = time.perf_counter()
     gw = pkg.World(world)
     t_upload = time.perf_counter() - t0
-    init = gw.state()
+    if eager:
+        gw.set_graphs(False)
+    init = _leg_state(gw, load)
     gw.step(2)  # first launches and graph capture
     gw.set_state(init)
     gw.synchronize()
+    if profile:
+        gw.set_profiling(True)
     el = 0.0
     for done in range(0, frames, 30):  # chunks with a progress line each
         k = min(30, frames - done)
@@ -282,6 +310,7 @@ def large_run(pkg, sc, n: int, frames: int):
         gw.step(k)
         el += time.perf_counter() - t0
         log(f"large scene: frames {done}-{done + k - 1} at {1e3 * el / (done + k):.1f} ms/frame so far")
+    kt = _leg_end(gw, save, profile)
     st = gw.state()
     out = {"workload": f"large single scene: {n} dynamic bodies (60% cubes, 40% ico42, scale U(0.5,1.5)) above a "
                        f"static floor, frames 0-{frames - 1}, dt=1/60, 20 substeps; uniform-grid broad phase, "
@@ -289,32 +318,42 @@ def large_run(pkg, sc, n: int, frames: int):
            "bodies": n, "frames": frames,
            "value": n * 20 * frames / el, "unit": "body-substeps/s", "ms_per_frame": 1e3 * el / frames,
            "scene_build_s": t_build, "upload_s": t_upload, "error_word": int(gw.errors()[0]),
-           "lowest_body_y": float(st["position"][1:, 1].min()), "active_bodies": int(st["active"][1:].sum())}
+           "lowest_body_y": float(st["position"][1:, 1].min()), "active_bodies": int(st["active"][1:].sum()),
+           "start": load or "frame 0", "kernel_times_ms": kt}
     gw.close()
     log("large scene: " + json.dumps(out))
     return out
 
 
-def uneven_run(pkg, sc, n_scenes: int, frames: int, tune_from: int):
+def uneven_run(pkg, sc, n_scenes: int, frames: int, tune_from: int, autotune=True, eager=False, load=None,
+               save=None, profile=False):
     """Batched uneven workload: n_scenes scenes of 4-256 bodies with 8/42/162-
     vertex hulls, 30% carrying joint chains; the autotuner picks the
     schedules on frames from a mid-run state; body-substeps/s over `frames`
-    frames from the start (graph replay)."""
+    frames from the start (graph replay). load / save / eager / profile as
+    for large_run (autotune off keeps the world's default schedules)."""
     t0 = time.perf_counter()
     world = sc.batched_uneven(n_scenes)
     t_build = time.perf_counter() - t0
     gw = pkg.World(world)
-    init = gw.state()
-    gw.step(tune_from)
-    tune = gw.autotune(frames=4)
-    gw.set_state(init)
+    if eager:
+        gw.set_graphs(False)
+    init = _leg_state(gw, load)
+    tune = None
+    if autotune:
+        gw.step(tune_from)
+        tune = gw.autotune(frames=4)
+        gw.set_state(init)
     gw.step(2)
     gw.set_state(init)
     gw.synchronize()
+    if profile:
+        gw.set_profiling(True)
     t0 = time.perf_counter()
     gw.step(frames, sync=False)
     gw.synchronize()
     el = time.perf_counter() - t0
+    kt = _leg_end(gw, save, profile)
     dyn = world.dynamic_bodies()
     errs = gw.errors()
     out = {"workload": f"batched uneven: {n_scenes} scenes, 4-256 bodies (log-uniform), 8/42/162-vertex hulls, "
@@ -323,7 +362,8 @@ def uneven_run(pkg, sc, n_scenes: int, frames: int, tune_from: int):
            "value": dyn * 20 * frames / el, "unit": "body-substeps/s", "ms_per_frame": 1e3 * el / frames,
            "schedule": {"narrow": gw.narrow_group, "manifold": gw.manifold_group, "solve": gw.solve_lanes,
                         "views": gw.views},
-           "autotune_ms": tune, "scene_build_s": t_build, "scenes_with_error_words": int((errs != 0).sum())}
+           "autotune_ms": tune, "scene_build_s": t_build, "scenes_with_error_words": int((errs != 0).sum()),
+           "start": load or "frame 0", "kernel_times_ms": kt}
     gw.close()
     log("uneven: " + json.dumps(out))
     return out
@@ -388,6 +428,12 @@ def main():
                     help="only the batched window (profiling runs): no whole run, no other workloads")
     ap.add_argument("--only", choices=("uneven", "large", "adjoint"),
                     help="run only this workload's leg (profiling runs) and print its result")
+    ap.add_argument("--save-state", metavar="NPZ",
+                    help="save the body state: after the window's start frame (batched), or at the end of an --only leg")
+    ap.add_argument("--load-state", metavar="NPZ",
+                    help="start from a saved body state: the batched window, or an --only leg (profiling windows)")
+    ap.add_argument("--leg-profile", action="store_true",
+                    help="with --only: HIP-event times of every kernel launch of the leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run one device-to-device copy of MIB MiB (a kernel of known traffic that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
@@ -417,8 +463,10 @@ def main():
     if args.copy_calibration:
         copy_calibration(args.copy_calibration << 20)
     if args.only:
-        leg = {"uneven": lambda: uneven_run(pkg, sc, args.uneven_scenes, args.uneven_frames, args.tune_from),
-               "large": lambda: large_run(pkg, sc, args.large_bodies, args.large_frames),
+        kw = {"eager": args.eager, "load": args.load_state, "save": args.save_state, "profile": args.leg_profile}
+        leg = {"uneven": lambda: uneven_run(pkg, sc, args.uneven_scenes, args.uneven_frames, args.tune_from,
+                                            autotune=not args.no_autotune, **kw),
+               "large": lambda: large_run(pkg, sc, args.large_bodies, args.large_frames, **kw),
                "adjoint": lambda: adjoint_run(pkg, sc, args.adjoint_scenes, args.adjoint_frames)}[args.only]
         print(json.dumps({"only": args.only, "result": leg()}), flush=True)
         return
@@ -459,7 +507,12 @@ def main():
     gw.step(2)
     gw.set_state(init)
     gw.synchronize()
-    if not args.window_only and args.run_frames >= args.start_frame:
+    if args.load_state:
+        with np.load(args.load_state, allow_pickle=False) as z:
+            gw.set_state({k: z[k] for k in z.files})
+        start_state = gw.state()
+        log(f"window start state loaded from {args.load_state}")
+    elif not args.window_only and args.run_frames >= args.start_frame:
         t0 = time.perf_counter()
         gw.step(args.start_frame, dt=dt, substeps=substeps, sync=False)
         gw.synchronize()
@@ -475,6 +528,9 @@ def main():
     else:
         gw.step(args.start_frame, dt=dt, substeps=substeps)
         start_state = gw.state()
+    if args.save_state:
+        np.savez(args.save_state, **start_state)
+        log(f"window start state saved to {args.save_state}")
 
     # the timed window: frames start_frame .. start_frame + steps - 1
     if args.warmup > 0:
