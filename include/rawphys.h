@@ -370,7 +370,7 @@ int32_t rp_world_set_solve_lanes(rp_world* w, int32_t lanes_per_scene);
 /* Large scenes (coloured solve): the first `classes` colour classes are
  * solved one launch each over the whole device, the remaining colours and
  * the serial class by one 1024-thread block with a barrier between classes
- * (0..64; default 16). Items of one colour are independent, so results are
+ * (0..64; default 64, every class). Items of one colour are independent, so results are
  * identical. Returns 0 or -1. */
 int32_t rp_world_set_colour_launches(rp_world* w, int32_t classes);
 int32_t rp_world_solve_lanes(const rp_world* w);

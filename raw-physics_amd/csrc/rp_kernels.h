@@ -103,7 +103,11 @@ constexpr int BIG_COLOURS = 63;             // parallel colours (one 64-bit mask
 constexpr int BIG_SERIAL = BIG_COLOURS;     // class solved by one lane in item order
 constexpr int BIG_CLASSES = BIG_COLOURS + 1;
 constexpr int BIG_COLOUR_ROUNDS = 64;       // Jones-Plassmann rounds per frame (launches end at once when all are coloured)
-constexpr int BIG_LAUNCH_CLASSES = 16;      // default: colours launched one by one before the one-block tail
+// Default: every class launched one by one over the device (a settled
+// 10^5-body pile leaves about 100 000 items beyond the first 16 colours,
+// 4 ms per substep on the one-block tail; an empty class's launch costs a
+// few microseconds).
+constexpr int BIG_LAUNCH_CLASSES = BIG_CLASSES;
 constexpr int LARGE_CHUNK = 256;            // bodies per wave in a large body's pair scan
 struct BigScene {
     int s, b0, n;                // scene, first body, bodies

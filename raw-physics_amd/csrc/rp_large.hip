@@ -287,13 +287,21 @@ void big_broad(const DevWorld& w, const BigScene& b, hipStream_t st) {
 __device__ __forceinline__ int uf_load(const int32_t* p, int x) {
     return __hip_atomic_load(p + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ int uf_root(const int32_t* p, int x) {
-    int y = uf_load(p, x);
-    while (y != x) {
-        x = y;
-        y = uf_load(p, x);
+// Root of x, halving the path on the way (x's parent becomes its
+// grandparent). Only non-roots are written, and hooks only change roots, so
+// the stores never race with a hook; every parent stays an ancestor with a
+// smaller or equal index, so the partition and the walks' termination are
+// unchanged. A settled pile is one island of 10^5 bodies: without the
+// halving its walks took the union pass to 2.7 ms per frame.
+__device__ __forceinline__ int uf_root(int32_t* p, int x) {
+    while (true) {
+        int y = uf_load(p, x);
+        if (y == x) return x;
+        int z = uf_load(p, y);
+        if (z == y) return y;
+        __hip_atomic_store(p + x, z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        x = z;
     }
-    return x;
 }
 // Links the sets of a and b: the larger root is hooked under the smaller, so
 // parent indices only decrease and every root walk ends.
@@ -334,7 +342,7 @@ __global__ void k_big_uf_union(DevWorld w, BigScene b) {
 
 // Sleep timers (pbd.cpp:483-495) and the island's all-asleep flag.
 __global__ void k_big_sleep(DevWorld w, BigScene b, double dt) {
-    const int32_t* parent = w.uf_parent + b.b0;
+    int32_t* parent = w.uf_parent + b.b0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < b.n; i += gridDim.x * blockDim.x) {
         Body& e = w.body[b.b0 + i];
         if (e.fixed) continue;
@@ -346,7 +354,7 @@ __global__ void k_big_sleep(DevWorld w, BigScene b, double dt) {
 }
 
 __global__ void k_big_active(DevWorld w, BigScene b) {
-    const int32_t* parent = w.uf_parent + b.b0;
+    int32_t* parent = w.uf_parent + b.b0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < b.n; i += gridDim.x * blockDim.x) {
         Body& e = w.body[b.b0 + i];
         if (e.fixed) continue;
