@@ -515,8 +515,13 @@ static bool build_image(const HostWorld& hw, DeviceImage& im) {
     d.stage_small = 2 * d.small_nv;
     im.pair_capacity = pcap;
     im.pair_base = pair_base;
+    // a few small scenes (the drop-in's one scene per call) leave the device
+    // nearly empty, so a pair's latency sets the step: 16 lanes per pair
+    // (1.09 ms per 16-body frame against 1.17 ms with 8, at a resting state);
+    // many scenes fill the device and 8 lanes per pair fit more pairs per wave
+    bool few = d.ns < 64 && im.big.empty();
     d.narrow_group = 1;
-    for (int g : {8, 16, 4})
+    for (int g : {few ? 16 : 8, few ? 8 : 16, 4})
         if (narrow_group_supported(d, g)) { d.narrow_group = g; break; }
     d.narrow_blocks = narrow_blocks_for(d, d.narrow_group, pcap);
     // manifolds built by the GJK + EPA kernel on its staged hulls where it
@@ -524,7 +529,7 @@ static bool build_image(const HostWorld& hw, DeviceImage& im) {
     d.manifold_group = d.narrow_group > 1 ? 0 : 1;
     d.manifold_blocks = narrow_blocks_for(d, 1, pcap);
     if (!im.ensure_group_work()) return false;
-    d.solve_lanes = 64;
+    d.solve_lanes = few ? 32 : 64;
     d.big_launch_classes = BIG_LAUNCH_CLASSES;
 
     bool ok = true;
