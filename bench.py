@@ -284,6 +284,63 @@ def _leg_end(gw, save, profile):
     return kt
 
 
+def plain_run(pkg, sc, world, start_state: dict, n_scenes: int, frames: int, substeps: int, dt: float,
+              eager=False, profile=False) -> dict:
+    """The plain method: one scene per step call, each frame's state written
+    back to the host (the drop-in pbd_simulate's cost model), every scene
+    from its state in `start_state` (the batched window's start).
+    eager: kernels launched one by one; profile: HIP events around every
+    launch of the first scene's frames (a second pass, not timed)."""
+    worlds = []
+    for s in range(n_scenes):
+        w1 = sc.World(name="plain")
+        sc.drop_scene(w1, s)
+        worlds.append(pkg.World(w1))
+    for w1 in worlds:  # warm
+        if eager:
+            w1.set_graphs(False)
+        w1.step(1)
+    for s, w1 in enumerate(worlds):
+        w1.set_state(_scene_slice(world, start_state, s, 1))
+    t0 = time.perf_counter()
+    for w1 in worlds:
+        for _ in range(frames):
+            w1.step(1, dt=dt, substeps=substeps)
+            w1.state()  # write-back to the host, as the drop-in does every call
+    pt = time.perf_counter() - t0
+    out = {"value": 16 * n_scenes * substeps * frames / pt, "unit": "body-substeps/s",
+           "ms_per_call": 1e3 * pt / (n_scenes * frames), "scenes": n_scenes, "frames": frames,
+           "schedule": {"narrow": worlds[0].narrow_group, "manifold": worlds[0].manifold_group,
+                        "solve": worlds[0].solve_lanes}}
+    log(f"plain: {n_scenes} scenes x {frames} frames one scene per call -> {out['value']:.4g}")
+    if profile:
+        w1 = worlds[0]
+        w1.set_state(_scene_slice(world, start_state, 0, 1))
+        w1.set_profiling(True)
+        w1.step(frames, dt=dt, substeps=substeps)
+        out["kernel_times_ms"] = w1.kernel_times()
+        w1.set_profiling(False)
+    for w1 in worlds:
+        w1.close()
+    return out
+
+
+def _plain_only(pkg, sc, args, substeps: int, dt: float) -> dict:
+    """The plain leg alone, from a batched window-start state saved with
+    --save-state (--load-state), or from frame 0."""
+    world = sc.batched_uniform(args.plain_scenes)
+    if args.load_state:
+        with np.load(args.load_state, allow_pickle=False) as z:
+            st = {k: z[k] for k in z.files}
+        log(f"window start state loaded from {args.load_state}")
+    else:
+        gw = pkg.World(world)
+        st = gw.state()
+        gw.close()
+    return plain_run(pkg, sc, world, st, args.plain_scenes, args.plain_frames, substeps, dt,
+                     eager=args.eager, profile=args.leg_profile)
+
+
 def large_run(pkg, sc, n: int, frames: int, eager=False, load=None, save=None, profile=False):
     """One large scene (grid broad phase, coloured solve): body-substeps/s.
     load / save: start from / keep a saved state (profiling windows deep in
@@ -426,7 +483,7 @@ def main():
     ap.add_argument("--large-frames", type=int, default=300)
     ap.add_argument("--window-only", action="store_true",
                     help="only the batched window (profiling runs): no whole run, no other workloads")
-    ap.add_argument("--only", choices=("uneven", "large", "adjoint"),
+    ap.add_argument("--only", choices=("uneven", "large", "adjoint", "plain"),
                     help="run only this workload's leg (profiling runs) and print its result")
     ap.add_argument("--save-state", metavar="NPZ",
                     help="save the body state: after the window's start frame (batched), or at the end of an --only leg")
@@ -467,7 +524,8 @@ def main():
         leg = {"uneven": lambda: uneven_run(pkg, sc, args.uneven_scenes, args.uneven_frames, args.tune_from,
                                             autotune=not args.no_autotune, **kw),
                "large": lambda: large_run(pkg, sc, args.large_bodies, args.large_frames, **kw),
-               "adjoint": lambda: adjoint_run(pkg, sc, args.adjoint_scenes, args.adjoint_frames)}[args.only]
+               "adjoint": lambda: adjoint_run(pkg, sc, args.adjoint_scenes, args.adjoint_frames),
+               "plain": lambda: _plain_only(pkg, sc, args, substeps, dt)}[args.only]
         print(json.dumps({"only": args.only, "result": leg()}), flush=True)
         return
     t0 = time.perf_counter()
@@ -567,25 +625,7 @@ def main():
 
     plain = None
     if not args.no_plain:
-        worlds = []
-        for s in range(args.plain_scenes):
-            w1 = sc.World(name="plain")
-            sc.drop_scene(w1, s)
-            worlds.append(pkg.World(w1))
-        for w1 in worlds:  # warm
-            w1.step(1)
-        for s, w1 in enumerate(worlds):  # each scene at the window's start state
-            w1.set_state(_scene_slice(world, start_state, s, 1))
-        t0 = time.perf_counter()
-        for w1 in worlds:
-            for _ in range(args.plain_frames):
-                w1.step(1, dt=dt, substeps=substeps)
-                w1.state()  # write-back to the host, as the drop-in does every call
-        pt = time.perf_counter() - t0
-        plain = 16 * args.plain_scenes * substeps * args.plain_frames / pt
-        log(f"plain: {args.plain_scenes} scenes x {args.plain_frames} frames one scene per call -> {plain:.4g}")
-        for w1 in worlds:
-            w1.close()
+        plain = plain_run(pkg, sc, world, start_state, args.plain_scenes, args.plain_frames, substeps, dt)
 
     cpu = None
     if not args.no_cpu:
@@ -674,8 +714,8 @@ def main():
                    "narrow_lanes_per_pair": narrow_group, "manifold_lanes_per_pair": manifold_group,
                    "solve_lanes_per_scene": solve_lanes, "views": views, "autotune_ms": tune},
         "whole_run": run,
-        "plain": {"value": plain, "unit": "body-substeps/s", "window": window,
-                  "ratio_batched_over_plain": (value / plain) if plain else None},
+        "plain": None if plain is None else {**plain, "window": window,
+                                             "ratio_batched_over_plain": value / plain["value"]},
         "adjoint": adjoint,
         "large_scene": large,
         "uneven": uneven,

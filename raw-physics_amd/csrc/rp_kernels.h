@@ -81,6 +81,7 @@ struct DevWorld {
     // solver scratch
     int32_t* item_level;            // [items]       level of each solve item (frame)
     int32_t* item_order;            // [items]       items grouped by level
+    int2* item_bodies;              // [items]       global body indices of each item_order entry
     int32_t* item_lstart;           // [items + ns]  level starts per scene
     int32_t* item_cursor;           // [items + ns]  fill cursors of the sort
     int32_t* scene_nlevels;         // [ns]

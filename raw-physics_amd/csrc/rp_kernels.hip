@@ -572,9 +572,12 @@ __global__ void __launch_bounds__(64) k_levels(DevWorld w, int collisions) {
     __syncthreads();
     for (int l = lane; l <= L; l += 64) cur[l] = ls[l];
     __syncthreads();
+    int2* bodies = w.item_bodies + ib;
     for (int t = lane; t < K; t += 64) {
         int pos = atomicAdd(&cur[lev[t]], 1);
-        order[pos] = (t < next) ? ~(e0 + t) : (g0 + t - next);
+        int code = (t < next) ? ~(e0 + t) : (g0 + t - next);
+        order[pos] = code;
+        bodies[pos] = item_body_pair(w, code, b0);
     }
 }
 
@@ -591,6 +594,7 @@ __global__ void __launch_bounds__(64) k_solve(DevWorld w, double h, int iters, i
     bool valid = s < w.ns && !w.scene_big[s];  // large scenes: coloured solve (rp_large.hip)
     int b0 = 0, n = 0, e0 = 0, e1x = 0, L = 0;
     const int* order = nullptr;
+    const int2* bodies = nullptr;
     const int* ls = nullptr;
     if (valid) {
         b0 = w.scene_body_off[s];
@@ -599,12 +603,22 @@ __global__ void __launch_bounds__(64) k_solve(DevWorld w, double h, int iters, i
         e1x = w.scene_ext_off[s + 1];
         int ib = w.scene_item_base[s];
         order = w.item_order + ib;
+        bodies = w.item_bodies + ib;
         ls = w.item_lstart + ib + s;
         L = w.scene_nlevels[s];
     }
     int Lmax = L;  // levels of the wave's busiest scene: every lane joins every barrier
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) Lmax = max(Lmax, __shfl_xor(Lmax, off));
+    // level starts ls[0..L] in the scene's lanes' registers (lane l holds
+    // ls[l]) when every scene of the wave fits, read back by lane exchange
+    // instead of a global load per level (the branch is uniform per wave)
+    bool ls_in_lanes = Lmax + 1 <= LPS;
+    int my_ls = (valid && lane <= L) ? ls[lane] : 0;
+    auto level_start = [&](int l) -> int {
+        if (ls_in_lanes) return __shfl(my_ls, l, LPS);
+        return (l <= L) ? ls[l] : 0;
+    };
     uint32_t err = 0;
     (void)collisions;
 
@@ -619,8 +633,9 @@ __global__ void __launch_bounds__(64) k_solve(DevWorld w, double h, int iters, i
     __syncthreads();
     for (int it = 0; it < iters; ++it) {
         for (int l = 0; l < Lmax; ++l) {
-            int i0 = (l < L) ? ls[l] : 0, i1 = (l < L) ? ls[l + 1] : 0;
-            for (int idx = i0 + lane; idx < i1; idx += LPS) solve_item_pos<kExt>(w, order[idx], b0, h, err);
+            int a = level_start(l), b = level_start(l + 1);
+            int i0 = (l < L) ? a : 0, i1 = (l < L) ? b : 0;
+            for (int idx = i0 + lane; idx < i1; idx += LPS) solve_item_pos<kExt>(w, order[idx], bodies[idx], h, err);
             __syncthreads();
         }
     }
@@ -629,8 +644,9 @@ __global__ void __launch_bounds__(64) k_solve(DevWorld w, double h, int iters, i
     __syncthreads();
     // velocity solve over the collision items, in the same levels
     for (int l = 0; l < Lmax; ++l) {
-        int i0 = (l < L) ? ls[l] : 0, i1 = (l < L) ? ls[l + 1] : 0;
-        for (int idx = i0 + lane; idx < i1; idx += LPS) solve_item_vel(w, order[idx], b0, h);
+        int a = level_start(l), b = level_start(l + 1);
+        int i0 = (l < L) ? a : 0, i1 = (l < L) ? b : 0;
+        for (int idx = i0 + lane; idx < i1; idx += LPS) solve_item_vel(w, order[idx], bodies[idx], h);
         __syncthreads();
     }
     if (err && valid) atomicOr(&w.scene_err[s], err);

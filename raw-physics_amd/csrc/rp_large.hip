@@ -434,7 +434,7 @@ __device__ __forceinline__ int big_items(const DevWorld& w, const BigScene& b, i
     return next + (collisions ? w.live_count[b.s] : 0);
 }
 // Scene-local non-fixed bodies of item t (-1 for a fixed one).
-__device__ __forceinline__ int2 item_bodies(const DevWorld& w, const BigScene& b, int t) {
+__device__ __forceinline__ int2 item_free_bodies(const DevWorld& w, const BigScene& b, int t) {
     int e0 = w.scene_ext_off[b.s];
     int next = w.scene_ext_off[b.s + 1] - e0;
     int i1, i2;
@@ -469,7 +469,7 @@ __global__ void k_big_csr_count(DevWorld w, BigScene b, int collisions) {
     if (K > b.item_cap) K = b.item_cap;
     int* colour = w.item_level + b.item_base;
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < K; t += gridDim.x * blockDim.x) {
-        int2 bb = item_bodies(w, b, t);
+        int2 bb = item_free_bodies(w, b, t);
         if (bb.x >= 0) atomicAdd(&b.bi_cursor[bb.x], 1);
         if (bb.y >= 0 && bb.y != bb.x) atomicAdd(&b.bi_cursor[bb.y], 1);
         colour[t] = -1;
@@ -483,7 +483,7 @@ __global__ void k_big_csr_count(DevWorld w, BigScene b, int collisions) {
 __global__ void k_big_csr_fill(DevWorld w, BigScene b, int collisions) {
     int K = min(big_items(w, b, collisions), b.item_cap);
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < K; t += gridDim.x * blockDim.x) {
-        int2 bb = item_bodies(w, b, t);
+        int2 bb = item_free_bodies(w, b, t);
         if (bb.x >= 0) b.bi_items[atomicAdd(&b.bi_cursor[bb.x], 1)] = t;
         if (bb.y >= 0 && bb.y != bb.x) b.bi_items[atomicAdd(&b.bi_cursor[bb.y], 1)] = t;
     }
@@ -509,7 +509,7 @@ __global__ void k_big_colour_round(DevWorld w, BigScene b, int collisions, const
             continue;
         }
         uint32_t pt = item_priority(t);
-        int2 bb = item_bodies(w, b, t);
+        int2 bb = item_free_bodies(w, b, t);
         bool best = true;
         uint64_t used = 0;
         for (int side = 0; side < 2 && best; ++side) {
@@ -599,7 +599,11 @@ __global__ void k_big_order(DevWorld w, BigScene b, int collisions) {
             if (c == cl) pos = base + __popcll(m & below);
             todo &= ~m;
         }
-        if (valid) order[pos] = item_code(w, b, t);
+        if (valid) {
+            int code = item_code(w, b, t);
+            order[pos] = code;
+            w.item_bodies[b.item_base + pos] = item_body_pair(w, code, b.b0);
+        }
     }
 }
 
@@ -648,19 +652,20 @@ template <bool kExt, bool kVel>
 __global__ void __launch_bounds__(64) k_big_class(DevWorld w, BigScene b, double h, int c) {
     const int* ls = w.item_lstart + b.item_base + b.s;
     const int* order = w.item_order + b.item_base;
+    const int2* bodies = w.item_bodies + b.item_base;
     int i0 = ls[c], i1 = ls[c + 1];
     uint32_t err = 0;
     if (c == BIG_SERIAL) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             for (int idx = i0; idx < i1; ++idx) {
-                if (kVel) solve_item_vel(w, order[idx], b.b0, h);
-                else solve_item_pos<kExt>(w, order[idx], b.b0, h, err);
+                if (kVel) solve_item_vel(w, order[idx], bodies[idx], h);
+                else solve_item_pos<kExt>(w, order[idx], bodies[idx], h, err);
             }
         }
     } else {
         for (int idx = i0 + blockIdx.x * blockDim.x + threadIdx.x; idx < i1; idx += gridDim.x * blockDim.x) {
-            if (kVel) solve_item_vel(w, order[idx], b.b0, h);
-            else solve_item_pos<kExt>(w, order[idx], b.b0, h, err);
+            if (kVel) solve_item_vel(w, order[idx], bodies[idx], h);
+            else solve_item_pos<kExt>(w, order[idx], bodies[idx], h, err);
         }
     }
     if (err) atomicOr(&w.scene_err[b.s], err);
@@ -672,20 +677,21 @@ template <bool kExt, bool kVel>
 __global__ void __launch_bounds__(1024) k_big_tail(DevWorld w, BigScene b, double h, int c0) {
     const int* ls = w.item_lstart + b.item_base + b.s;
     const int* order = w.item_order + b.item_base;
+    const int2* bodies = w.item_bodies + b.item_base;
     uint32_t err = 0;
     for (int c = c0; c < BIG_COLOURS; ++c) {
         int i0 = ls[c], i1 = ls[c + 1];
         if (i0 == i1) continue;  // the same for every thread
         for (int idx = i0 + (int)threadIdx.x; idx < i1; idx += blockDim.x) {
-            if (kVel) solve_item_vel(w, order[idx], b.b0, h);
-            else solve_item_pos<kExt>(w, order[idx], b.b0, h, err);
+            if (kVel) solve_item_vel(w, order[idx], bodies[idx], h);
+            else solve_item_pos<kExt>(w, order[idx], bodies[idx], h, err);
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
         for (int idx = ls[BIG_SERIAL]; idx < ls[BIG_SERIAL + 1]; ++idx) {
-            if (kVel) solve_item_vel(w, order[idx], b.b0, h);
-            else solve_item_pos<kExt>(w, order[idx], b.b0, h, err);
+            if (kVel) solve_item_vel(w, order[idx], bodies[idx], h);
+            else solve_item_pos<kExt>(w, order[idx], bodies[idx], h, err);
         }
     }
     if (err) atomicOr(&w.scene_err[b.s], err);

@@ -581,6 +581,7 @@ static bool build_image(const HostWorld& hw, DeviceImage& im) {
     ok &= im.alloc(d.ext_lambda, 3 * hw.ext.size());
     ok &= im.alloc(d.item_level, (size_t)ccap);
     ok &= im.alloc(d.item_order, (size_t)ccap);
+    ok &= im.alloc(d.item_bodies, (size_t)ccap);
     ok &= im.alloc(d.item_lstart, (size_t)ccap + d.ns);
     ok &= im.alloc(d.item_cursor, (size_t)ccap + d.ns);
     ok &= im.alloc(d.scene_nlevels, d.ns);
