@@ -181,6 +181,7 @@ __global__ void k_scatter_live(DevWorld w) {
             int2 p = w.pairs[slot];
             b1 = b0 + p.x;
             b2 = b0 + p.y;
+            w.live_rec[g0 + m] = make_int4(b1, b2, w.body_hull[b1], w.body_hull[b2]);
             w.need[b1] = 1;  // same value from every writer
             w.need[b2] = 1;
         }
@@ -304,14 +305,13 @@ __device__ __forceinline__ void group_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-// Scene and global body indices of live pair g.
+// Scene and global body indices of live pair g (both loads depend on g
+// only: the bodies come from the frame's live-pair record).
 __device__ __forceinline__ void pair_bodies(const DevWorld& w, int g, int& s, int& b1, int& b2) {
-    int2 sp = w.live_global[g];
-    s = sp.x;
-    int2 p = w.pairs[sp.y];
-    int b0 = w.scene_body_off[s];
-    b1 = b0 + p.x;
-    b2 = b0 + p.y;
+    s = w.live_global[g].x;
+    int4 r = w.live_rec[g];
+    b1 = r.x;
+    b2 = r.y;
 }
 // Pairs with a sphere or a multi-collider body take the general path.
 __device__ __forceinline__ bool general_pair(const DevWorld& w, int b1, int b2) {
@@ -323,17 +323,15 @@ __device__ __forceinline__ bool general_pair(const DevWorld& w, int b1, int b2) 
 template <int G>
 __device__ __forceinline__ void pair_views(const DevWorld& w, int g, V3* st, int gl, int& s, int& b1, int& b2,
                                            HullView& h1, HullView& h2) {
-    int2 sp = w.live_global[g];
-    s = sp.x;
-    int2 p = w.pairs[sp.y];
-    int b0 = w.scene_body_off[s];
-    b1 = b0 + p.x;
-    b2 = b0 + p.y;
-    h1 = hull_view(w, b1);
-    h2 = hull_view(w, b2);
+    s = w.live_global[g].x;
+    int4 r = w.live_rec[g];  // bodies and hull instances: no pair-slot and scene-offset loads first
+    b1 = r.x;
+    b2 = r.y;
+    h1 = hull_view_of(w, b1, r.z, w.body_tv_off[b1]);
+    h2 = hull_view_of(w, b2, r.w, w.body_tv_off[b2]);
     if constexpr (G > 1) {
-        const V3* lv1 = w.hull_v + w.hull_v_off[w.body_hull[b1]];
-        const V3* lv2 = w.hull_v + w.hull_v_off[w.body_hull[b2]];
+        const V3* lv1 = w.hull_v + w.hull_v_off[r.z];
+        const V3* lv2 = w.hull_v + w.hull_v_off[r.w];
         const M4 M1 = w.model[b1], M2 = w.model[b2];
         // both hulls together, two vertices of each per lane and batch, all
         // loaded before any is transformed
