@@ -97,6 +97,11 @@ constexpr int EPA_MAXF = 256;
 constexpr int EPA_MAXE = 128;
 constexpr int CLIP_MAX = 64;
 constexpr int MAXC = 16;       // contacts kept per body pair
+// Index of a first-maximum / first-minimum search that has found nothing:
+// larger than every real index, so "ties to the lower index" also prefers
+// a lane that found something (its value is then strictly better, since
+// the search starts from the most extreme finite value).
+constexpr int NO_SEL = 0x7fffffff;
 
 // ---------------------------------------------------------------------------
 // Support mapping (support.cpp:5-39): first index wins on ties
@@ -112,21 +117,24 @@ constexpr int MAXC = 16;       // contacts kept per body pair
 // (quad permutations), 7 - lane within 8 (half mirror), 15 - lane (mirror).
 enum : int { DPP_XOR1 = 0xB1, DPP_XOR2 = 0x4E, DPP_HALF_MIRROR = 0x141, DPP_MIRROR = 0x140 };
 template <int CTRL> __device__ __forceinline__ int dpp_int(int v) {
-    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, true);
 }
 template <int CTRL> __device__ __forceinline__ double dpp_double(double v) {
     long long b = __double_as_longlong(v);
     int lo = dpp_int<CTRL>((int)b), hi = dpp_int<CTRL>((int)(b >> 32));
     return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
 }
-// (dot, index) of the partner lane, kept if its dot is larger or equal with a
-// lower index; an index < 0 means "none yet".
+// (dot, index) of the partner lane, kept if its dot is larger, or equal
+// with a lower index. A lane that has found nothing holds (-DBL_MAX,
+// NO_SEL); a lane that has found something holds a dot above -DBL_MAX (its
+// scan took only values above it), so one comparison of each kind decides
+// every case.
 // Written as selects over a non-short-circuit predicate, so that it compiles
 // to compares and v_cndmask instead of a branch per term.
 template <int CTRL> __device__ __forceinline__ void argmax_step(double& m, int& sel) {
     double om = dpp_double<CTRL>(m);
     int os = dpp_int<CTRL>(sel);
-    bool take = (os >= 0) & ((sel < 0) | (om > m) | ((om == m) & (os < sel)));
+    bool take = (om > m) | ((om == m) & (os < sel));
     m = take ? om : m;
     sel = take ? os : sel;
 }
@@ -139,13 +147,30 @@ template <int G> __device__ __forceinline__ void group_argmax(double& m, int& se
     if constexpr (G >= 8) argmax_step<DPP_HALF_MIRROR>(m, sel);
     if constexpr (G >= 16) argmax_step<DPP_MIRROR>(m, sel);
 }
+// Two independent first maxima, their steps interleaved so that the two
+// dependency chains overlap.
+template <int G> __device__ __forceinline__ void group_argmax2(double& m1, int& s1, double& m2, int& s2) {
+    static_assert(G == 4 || G == 8 || G == 16, "lane groups of 4, 8 or 16");
+    argmax_step<DPP_XOR1>(m1, s1);
+    argmax_step<DPP_XOR1>(m2, s2);
+    argmax_step<DPP_XOR2>(m1, s1);
+    argmax_step<DPP_XOR2>(m2, s2);
+    if constexpr (G >= 8) {
+        argmax_step<DPP_HALF_MIRROR>(m1, s1);
+        argmax_step<DPP_HALF_MIRROR>(m2, s2);
+    }
+    if constexpr (G >= 16) {
+        argmax_step<DPP_MIRROR>(m1, s1);
+        argmax_step<DPP_MIRROR>(m2, s2);
+    }
+}
 
 // (value, index) of the partner lane, kept if its value is smaller, or equal
-// with a lower index; an index < 0 means "none yet".
+// with a lower index; a lane that has found nothing holds (DBL_MAX, NO_SEL).
 template <int CTRL> __device__ __forceinline__ void argmin_step(double& m, int& sel) {
     double om = dpp_double<CTRL>(m);
     int os = dpp_int<CTRL>(sel);
-    bool take = (os >= 0) & ((sel < 0) | (om < m) | ((om == m) & (os < sel)));
+    bool take = (om < m) | ((om == m) & (os < sel));
     m = take ? om : m;
     sel = take ? os : sel;
 }
@@ -160,7 +185,7 @@ template <int G> __device__ __forceinline__ void group_argmin(double& m, int& se
 
 template <int G = 1>
 RP_HD int support_index(const HullView& h, V3 d, uint32_t& err) {
-    int sel = -1;
+    int sel = NO_SEL;
     double max_dot = -DBL_MAX;
     if constexpr (G == 1) {
         int i = 0;
@@ -188,7 +213,7 @@ RP_HD int support_index(const HullView& h, V3 d, uint32_t& err) {
         }
         group_argmax<G>(max_dot, sel);
     }
-    if (sel < 0) { err |= ERR_SUPPORT_NAN; sel = 0; }
+    if (sel == NO_SEL) { err |= ERR_SUPPORT_NAN; sel = 0; }
     return sel;
 }
 // Support indices of two hulls in directions d1 and d2 at once: the two
@@ -198,7 +223,7 @@ RP_HD int support_index(const HullView& h, V3 d, uint32_t& err) {
 // G > 1 the two group reductions run side by side.
 template <int G = 1>
 RP_HD void support_index2(const HullView& a, V3 d1, const HullView& b, V3 d2, int& i1, int& i2, uint32_t& err) {
-    int s1 = -1, s2 = -1;
+    int s1 = NO_SEL, s2 = NO_SEL;
     double m1 = -DBL_MAX, m2 = -DBL_MAX;
     int n = a.nv > b.nv ? a.nv : b.nv;
     if constexpr (G == 1) {
@@ -248,11 +273,10 @@ RP_HD void support_index2(const HullView& a, V3 d1, const HullView& b, V3 d2, in
                 m2 = tb ? db : m2;
             }
         }
-        group_argmax<G>(m1, s1);
-        group_argmax<G>(m2, s2);
+        group_argmax2<G>(m1, s1, m2, s2);
     }
-    if (s1 < 0) { err |= ERR_SUPPORT_NAN; s1 = 0; }
-    if (s2 < 0) { err |= ERR_SUPPORT_NAN; s2 = 0; }
+    if (s1 == NO_SEL) { err |= ERR_SUPPORT_NAN; s1 = 0; }
+    if (s2 == NO_SEL) { err |= ERR_SUPPORT_NAN; s2 = 0; }
     i1 = s1;
     i2 = s2;
 }
@@ -734,7 +758,7 @@ __device__ int epa_group(const View& h1, const View& h2, const Simplex& s, V3& n
         group_mem_sync();
         // the closest face: first minimum of the distances
         double md = DBL_MAX;
-        int mi = -1;
+        int mi = NO_SEL;
         for (int i = gl; i < nf; i += G) {
             double di = P.dist[i];
             bool t = di < md;
@@ -743,7 +767,7 @@ __device__ int epa_group(const View& h1, const View& h2, const Simplex& s, V3& n
         }
         group_argmin<G>(md, mi);
         min_d = md;
-        if (mi >= 0) min_n = P.normals[mi];
+        if (mi != NO_SEL) min_n = P.normals[mi];
     }
     return 0;
 }
@@ -954,7 +978,7 @@ __device__ __forceinline__ int best_face_group(int sidx, const HullView& h, V3 n
     int gl = (int)(__lane_id() & (G - 1));
     int b = h.vf_off[sidx], cnt = h.vf_off[sidx + 1] - b;
     double max_proj = -DBL_MAX;
-    int pos = -1;
+    int pos = NO_SEL;
     for (int k = gl; k < cnt; k += G) {
         double proj = dot(face_normal_world(h, h.vf[b + k]), n);
         bool t = proj > max_proj;
@@ -962,7 +986,7 @@ __device__ __forceinline__ int best_face_group(int sidx, const HullView& h, V3 n
         pos = t ? k : pos;
     }
     group_argmax<G>(max_proj, pos);
-    if (pos < 0) { err |= ERR_SELECT; return 0; }
+    if (pos == NO_SEL) { err |= ERR_SELECT; return 0; }
     return h.vf[b + pos];
 }
 
@@ -1098,7 +1122,7 @@ RP_HD int manifold(const HullView& h1, const HullView& h2, V3 normal, ContactPt*
         int i0 = h1.vn_off[s1], n1 = h1.vn_off[s1 + 1] - i0;
         int j0 = h2.vn_off[s2], n2 = h2.vn_off[s2 + 1] - j0;
         double max_dot = -DBL_MAX;
-        int best = -1;
+        int best = NO_SEL;
         for (int k = gl; k < n1 * n2; k += G) {
             int a = k / n2, b = k - a * n2;
             V3 ed1 = sub(sp1, h1.tv[h1.vn[i0 + a]]);
@@ -1115,7 +1139,7 @@ RP_HD int manifold(const HullView& h1, const HullView& h2, V3 normal, ContactPt*
             best = t ? 2 * k + 1 : best;
         }
         group_argmax<G>(max_dot, best);
-        if (best < 0) {
+        if (best == NO_SEL) {
             err |= ERR_SELECT;
         } else {
             int k = best >> 1, a = k / n2, b = k - a * n2;
