@@ -682,6 +682,7 @@ def main():
                                      f"{unit_ops[0] / max(unit_ops[1], 1):.0f} FP64 ops per {unit_ops[2]}"}
             log("roofline: " + json.dumps(roofline))
             if args.ops_out:
+                Path(args.ops_out).parent.mkdir(parents=True, exist_ok=True)
                 Path(args.ops_out).write_text(json.dumps(
                     {"window": f"batched uniform {args.scenes} scenes, frames {args.start_frame}-"
                                f"{args.start_frame + args.steps - 1} from the frame-{args.start_frame} state",
