@@ -35,6 +35,40 @@ __device__ __forceinline__ bool broad_hit(const Body& e1, const Body& e2) {
     return broad_hit(e1.x, e1.radius, e2.x, e2.radius);
 }
 
+// Lock-free union-find over scene-local parent indices (islands,
+// broad.cpp:31-116): any order of the unions gives the same partition.
+__device__ __forceinline__ int uf_load(const int32_t* p, int x) {
+    return __hip_atomic_load(p + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Root of x, halving the path on the way (x's parent becomes its
+// grandparent). Only non-roots are written, and hooks only change roots, so
+// the stores never race with a hook; every parent stays an ancestor with a
+// smaller or equal index, so the partition and the walks' termination are
+// unchanged. A settled pile is one island of 10^5 bodies: without the
+// halving its walks took the union pass to 2.7 ms per frame.
+__device__ __forceinline__ int uf_root(int32_t* p, int x) {
+    while (true) {
+        int y = uf_load(p, x);
+        if (y == x) return x;
+        int z = uf_load(p, y);
+        if (z == y) return y;
+        __hip_atomic_store(p + x, z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        x = z;
+    }
+}
+// Links the sets of a and b: the larger root is hooked under the smaller, so
+// parent indices only decrease and every root walk ends.
+__device__ __forceinline__ void uf_link(int32_t* p, int a, int b) {
+    while (true) {
+        a = uf_root(p, a);
+        b = uf_root(p, b);
+        if (a == b) return;
+        if (a > b) { int t = a; a = b; b = t; }
+        int old = atomicCAS(p + b, b, a);
+        if (old == b) return;
+    }
+}
+
 // Global body indices of a solve item: an external constraint (item < 0,
 // index ~item) or live pair `item` of the scene whose first body is b0.
 // Computed once per frame into item_bodies beside item_order, so that the

@@ -66,13 +66,12 @@ __global__ void k_broad(DevWorld w) {
     if (lane == 0) w.pair_count[s] = count;
 }
 
-static __device__ int uf_find(int* parent, int x) {
-    while (parent[x] != x) x = parent[x];
-    return x;
-}
-
 // Islands and sleeping (broad.cpp:31-116, pbd.cpp:476-506), then the list of
-// pairs the narrow phase visits (pbd.cpp:593-596).
+// pairs the narrow phase visits (pbd.cpp:593-596). The reference joins the
+// islands one pair after another (uf_union, broad.cpp:55-59); the wave's
+// lanes join them at once with the lock-free union-find, which gives the same
+// partition, and a body is active unless every body of its island has slept
+// for a second, which depends on the partition only.
 __global__ void k_islands(DevWorld w, double dt) {
     int s = blockIdx.x;
     if (w.scene_big[s]) return;
@@ -82,36 +81,30 @@ __global__ void k_islands(DevWorld w, double dt) {
     int np = w.pair_count[s];
     int* parent = w.uf_parent + b0;  // scene-local indices
     int* alli = w.uf_flag + b0;
-    if (lane == 0) {
-        for (int i = 0; i < n; ++i) { parent[i] = i; alli[i] = 1; }
-        // uf_union (broad.cpp:55-59): parent[find(y)] = find(x)
-        for (int k = 0; k < np; ++k) {
-            int2 p = w.pairs[base + k];
-            if (!w.body[b0 + p.x].fixed && !w.body[b0 + p.y].fixed) {
-                int key = uf_find(parent, p.y), value = uf_find(parent, p.x);
-                parent[key] = value;
-            }
-        }
-        for (int k = w.scene_ext_off[s]; k < w.scene_ext_off[s + 1]; ++k) {
-            int i = w.ext[k].b1 - b0, j = w.ext[k].b2 - b0;
-            if (!w.body[b0 + i].fixed && !w.body[b0 + j].fixed) {
-                int key = uf_find(parent, j), value = uf_find(parent, i);
-                parent[key] = value;
-            }
-        }
-        for (int i = 0; i < n; ++i) {
-            Body& e = w.body[b0 + i];
-            if (e.fixed) continue;
-            double lv = len(e.v), av = len(e.w);
-            if (lv < 0.10 && av < 0.10) e.deact += dt;
-            else e.deact = 0.0;
-            if (e.deact < 1.0) alli[uf_find(parent, i)] = 0;
-        }
-        for (int i = 0; i < n; ++i) {
-            Body& e = w.body[b0 + i];
-            if (e.fixed) continue;
-            e.active = !alli[uf_find(parent, i)];
-        }
+    for (int i = lane; i < n; i += 64) { parent[i] = i; alli[i] = 1; }
+    __syncthreads();
+    for (int k = lane; k < np; k += 64) {
+        int2 p = w.pairs[base + k];
+        if (!w.body[b0 + p.x].fixed && !w.body[b0 + p.y].fixed) uf_link(parent, p.x, p.y);
+    }
+    for (int k = w.scene_ext_off[s] + lane; k < w.scene_ext_off[s + 1]; k += 64) {
+        int i = w.ext[k].b1 - b0, j = w.ext[k].b2 - b0;
+        if (!w.body[b0 + i].fixed && !w.body[b0 + j].fixed) uf_link(parent, i, j);
+    }
+    __syncthreads();
+    for (int i = lane; i < n; i += 64) {
+        Body& e = w.body[b0 + i];
+        if (e.fixed) continue;
+        double lv = len(e.v), av = len(e.w);
+        if (lv < 0.10 && av < 0.10) e.deact += dt;
+        else e.deact = 0.0;
+        if (e.deact < 1.0) alli[uf_root(parent, i)] = 0;
+    }
+    __syncthreads();
+    for (int i = lane; i < n; i += 64) {
+        Body& e = w.body[b0 + i];
+        if (e.fixed) continue;
+        e.active = !alli[uf_root(parent, i)];
     }
     __syncthreads();
     // live pairs, in pair order

@@ -284,38 +284,6 @@ void big_broad(const DevWorld& w, const BigScene& b, hipStream_t st) {
 // ---------------------------------------------------------------------------
 // Islands and sleeping
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int uf_load(const int32_t* p, int x) {
-    return __hip_atomic_load(p + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-// Root of x, halving the path on the way (x's parent becomes its
-// grandparent). Only non-roots are written, and hooks only change roots, so
-// the stores never race with a hook; every parent stays an ancestor with a
-// smaller or equal index, so the partition and the walks' termination are
-// unchanged. A settled pile is one island of 10^5 bodies: without the
-// halving its walks took the union pass to 2.7 ms per frame.
-__device__ __forceinline__ int uf_root(int32_t* p, int x) {
-    while (true) {
-        int y = uf_load(p, x);
-        if (y == x) return x;
-        int z = uf_load(p, y);
-        if (z == y) return y;
-        __hip_atomic_store(p + x, z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        x = z;
-    }
-}
-// Links the sets of a and b: the larger root is hooked under the smaller, so
-// parent indices only decrease and every root walk ends.
-__device__ __forceinline__ void uf_link(int32_t* p, int a, int b) {
-    while (true) {
-        a = uf_root(p, a);
-        b = uf_root(p, b);
-        if (a == b) return;
-        if (a > b) { int t = a; a = b; b = t; }
-        int old = atomicCAS(p + b, b, a);
-        if (old == b) return;
-    }
-}
-
 __global__ void k_big_uf_init(DevWorld w, BigScene b) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < b.n; i += gridDim.x * blockDim.x) {
         w.uf_parent[b.b0 + i] = i;
