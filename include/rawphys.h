@@ -351,6 +351,12 @@ int32_t rp_world_manifold_group(const rp_world* w);
  * identical. Returns 0 or -1. */
 int32_t rp_world_set_narrow_block_cap(rp_world* w, int32_t blocks);
 
+/* Dynamic LDS bytes one narrow-phase block may use on the world's device
+ * (the device's limit per block: 160 KiB on gfx950). A lane-group schedule
+ * of G lanes per pair fits where 64 / G pairs' slices of 2 x (the world's
+ * largest hull's vertices) x 24 bytes do. */
+int32_t rp_world_lds_limit(const rp_world* w);
+
 /* Concurrent views: graph replay cuts the world into up to 4 scene ranges
  * of about equal body counts and replays each on a stream of its own, so
  * that one range's latency-bound kernels overlap another's (scenes never

@@ -68,6 +68,7 @@ def library():
         "rp_world_set_manifold_group": (i32, [_P, i32]),
         "rp_world_manifold_group": (i32, [_P]),
         "rp_world_set_narrow_block_cap": (i32, [_P, i32]),
+        "rp_world_lds_limit": (i32, [_P]),
         "rp_world_set_solve_lanes": (i32, [_P, i32]),
         "rp_world_set_colour_launches": (i32, [_P, i32]),
         "rp_world_solve_lanes": (i32, [_P]),
@@ -302,6 +303,11 @@ class World:
     @property
     def solve_lanes(self) -> int:
         return int(self.lib.rp_world_solve_lanes(self.h))
+
+    @property
+    def lds_limit(self) -> int:
+        """Dynamic LDS bytes one narrow-phase block may use on the device."""
+        return int(self.lib.rp_world_lds_limit(self.h))
 
     @property
     def narrow_group(self) -> int:

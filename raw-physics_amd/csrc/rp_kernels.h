@@ -24,6 +24,7 @@ struct DevWorld {
     int manifold_group;             // lanes per pair in the contact manifold: 1, 4, 8, 16
     int manifold_blocks;
     int narrow_block_cap;           // most blocks of a narrow-phase launch (0: the default, 8192)
+    int lds_limit;                  // dynamic LDS bytes one block may use on the world's device
     int stage_cap;                  // LDS vertices per pair: 2 x the largest hull
     int small_nv;                   // > 0: pairs of two hulls of <= small_nv vertices go to their own list
     int stage_small;                // LDS vertices per pair of that list: 2 x small_nv
@@ -216,6 +217,9 @@ constexpr size_t NOALLOC_SCRATCH_BYTES = 64u << 20;
 unsigned char* noalloc_scratch();
 // Lanes per pair the narrow phase can use for this world (LDS permitting).
 bool narrow_group_supported(const DevWorld& w, int g);
+// Lets the lane-group narrow-phase kernels take up to `bytes` of dynamic LDS
+// per block (gfx950 gives a workgroup up to 160 KiB).
+hipError_t allow_narrow_lds(int bytes);
 int narrow_blocks_for(const DevWorld& w, int g, int64_t pair_capacity);
 hipError_t launch_fp64_selftest(const double* a, const double* b, double* out, int64_t n, hipStream_t st);
 

@@ -873,7 +873,19 @@ int64_t group_work_slots_needed(const DevWorld& w) {
 bool narrow_group_supported(const DevWorld& w, int g) {
     if (g == 1) return true;
     if (g != 4 && g != 8 && g != 16) return false;
-    return (size_t)(64 / g) * w.stage_cap * sizeof(V3) <= 64 * 1024;
+    return (size_t)(64 / g) * w.stage_cap * sizeof(V3) <= (size_t)w.lds_limit;
+}
+
+hipError_t allow_narrow_lds(int bytes) {
+    const void* k[] = {(const void*)k_gjk_epa<4, true>, (const void*)k_gjk_epa<8, true>,
+                       (const void*)k_gjk_epa<16, true>, (const void*)k_gjk_epa<4, false>,
+                       (const void*)k_gjk_epa<8, false>, (const void*)k_gjk_epa<16, false>,
+                       (const void*)k_manifold<4>, (const void*)k_manifold<8>, (const void*)k_manifold<16>};
+    for (const void* f : k) {
+        hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 // Blocks of the narrow-phase kernels: one lane group per pair slot, up to

@@ -504,11 +504,21 @@ static bool build_image(const HostWorld& hw, DeviceImage& im) {
     int max_nv = 1;
     for (const HostHull& h : hw.hulls) max_nv = std::max(max_nv, (int)h.v.size());
     d.stage_cap = 2 * max_nv;
+    // LDS a narrow-phase block may stage hulls in: the device's limit per
+    // block (160 KiB on gfx950), 64 KiB if the query fails
+    {
+        int dev = 0, lim = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || lim <= 0)
+            lim = 64 * 1024;
+        d.lds_limit = lim;
+        if (lim > 64 * 1024) RP_CHECK(allow_narrow_lds(lim));
+    }
     // size classes: with hulls above SMALL_HULL_NV vertices, the pairs of two
     // smaller hulls get their own list and LDS slices sized for them (if the
     // large-hull pairs fit a 16-lane schedule)
     d.small_nv = 0;
-    if (max_nv > SMALL_HULL_NV && (size_t)(64 / 16) * d.stage_cap * sizeof(V3) <= 64 * 1024) {
+    if (max_nv > SMALL_HULL_NV && (size_t)(64 / 16) * d.stage_cap * sizeof(V3) <= (size_t)d.lds_limit) {
         for (const HostHull& h : hw.hulls)
             if ((int)h.v.size() <= SMALL_HULL_NV) d.small_nv = std::max(d.small_nv, (int)h.v.size());
     }
@@ -1501,6 +1511,8 @@ int32_t rp_world_set_manifold_group(rp_world* w, int32_t lanes_per_pair) {
 }
 
 int32_t rp_world_manifold_group(const rp_world* w) { return w->im.d.manifold_group; }
+
+int32_t rp_world_lds_limit(const rp_world* w) { return w->im.d.lds_limit; }
 
 int32_t rp_world_set_narrow_block_cap(rp_world* w, int32_t blocks) {
     if (blocks < 0 || blocks > 256 * 32) return -1;

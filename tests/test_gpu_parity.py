@@ -99,6 +99,23 @@ def test_batched_ico162_bit_exact():
     _pair(scenes.batched_uniform(3, ico_subdiv=2), frames=60, check_every=20)
 
 
+def test_642_vertex_hulls_on_lane_groups_bit_exact():
+    """Icospheres of 642 vertices beside cubes: a 16-lane group stages two
+    such hulls in 30.8 KiB, so four groups take 123 KiB of a block's LDS,
+    which gfx950 allows (160 KiB). The world runs its large-hull pairs on
+    the 16-lane schedule and its cube pairs on the 8-lane one, and matches
+    the oracle bit for bit."""
+    world = scenes.batched_uniform(2, first_seed=40, n_cubes=6, n_icos=3, ico_subdiv=3)
+    g = SimWorld("rp", world)
+    lim = g.lib.rp_world_lds_limit(g.h)
+    print(f"LDS limit per block: {lim} bytes")
+    if lim < 4 * 2 * 642 * 24:
+        pytest.skip(f"the device gives a block {lim} bytes of LDS")
+    assert g.lib.rp_world_narrow_group(g.h) > 1, "the large hulls should fit a lane-group schedule"
+    g.close()
+    _pair(world, frames=45, check_every=15)
+
+
 @pytest.mark.parametrize("kind", ["narrow", "manifold", "fused"])
 @pytest.mark.parametrize("lanes", [1, 4, 8, 16])
 def test_every_narrow_schedule_bit_exact(kind, lanes):
@@ -114,10 +131,11 @@ def test_every_narrow_schedule_bit_exact(kind, lanes):
         if kind == "fused":
             assert g.lib.rp_world_set_manifold_group(g.h, 0) == 0
         # a group stages both hulls of its pair in LDS; schedules whose
-        # (64 / lanes) groups would need more than 64 KiB are refused
+        # (64 / lanes) groups would need more than the device's LDS per block
+        # are refused
         max_nv = max(len(np.unique(c, axis=0)) for c in world.shapes)
         lds = (64 // lanes) * 2 * max_nv * 24 if lanes > 1 else 0
-        if lds > 65536:
+        if lds > g.lib.rp_world_lds_limit(g.h):
             assert getattr(g.lib, setter)(g.h, lanes) != 0
             continue
         assert getattr(g.lib, setter)(g.h, lanes) == 0
