@@ -278,6 +278,9 @@ uint32_t rp_world_step_noalloc(rp_world* w, double dt, uint32_t frames, uint32_t
 void rp_world_get_state(rp_world* w, const rp_state* out);
 void rp_world_set_state(rp_world* w, const rp_state* in);
 uint32_t rp_world_error(rp_world* w, uint32_t scene);
+/* Every scene's error word, in one copy, into out[num_scenes]. Returns 0, or
+ * RP_ERR_RUNTIME if the device copy failed. */
+uint32_t rp_world_errors(rp_world* w, uint32_t* out);
 /* EPA runs that did not converge since creation (epa.cpp:233-236). */
 uint64_t rp_world_epa_failures(rp_world* w);
 /* GJK runs (gjk.cpp:350-379) that were ended early on a repeated loop state

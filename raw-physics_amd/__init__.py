@@ -41,6 +41,7 @@ def library():
         "rp_world_get_state": (None, [_P, ctypes.POINTER(scenes.StateC)]),
         "rp_world_set_state": (None, [_P, ctypes.POINTER(scenes.StateC)]),
         "rp_world_error": (u32, [_P, u32]),
+        "rp_world_errors": (u32, [_P, _P]),
         "rp_world_epa_failures": (u64, [_P]),
         "rp_world_gjk_cycle_exits": (u64, [_P]),
         "rp_world_stream": (_P, [_P]),
@@ -163,7 +164,11 @@ class World:
         self.lib.rp_world_set_state(self.h, ctypes.byref(s))
 
     def errors(self) -> np.ndarray:
-        return np.array([self.lib.rp_world_error(self.h, s) for s in range(self.ns)], dtype=np.uint32)
+        """Every scene's error word."""
+        out = np.zeros(self.ns, dtype=np.uint32)
+        if self.ns and self.lib.rp_world_errors(self.h, out.ctypes.data) != 0:
+            raise RuntimeError("rp_world_errors failed: " + last_error_string())
+        return out
 
     def epa_failures(self) -> int:
         return int(self.lib.rp_world_epa_failures(self.h))

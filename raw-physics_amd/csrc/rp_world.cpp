@@ -1207,6 +1207,15 @@ uint32_t rp_world_error(rp_world* w, uint32_t scene) {
     return e;
 }
 
+uint32_t rp_world_errors(rp_world* w, uint32_t* out) {
+    uint32_t ns = rp_world_num_scenes(w);
+    if (!out || ns == 0) return 0;
+    if (hipStreamSynchronize(w->stream) != hipSuccess ||
+        hipMemcpy(out, w->im.d.scene_err, (size_t)ns * 4, hipMemcpyDeviceToHost) != hipSuccess)
+        return RP_ERR_RUNTIME;
+    return 0;
+}
+
 static uint64_t read_stat(rp_world* w, int k) {
     unsigned long long v = 0;
     if (hipStreamSynchronize(w->stream) != hipSuccess) return 0;
