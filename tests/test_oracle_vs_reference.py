@@ -68,3 +68,16 @@ def test_random_pair_contacts_bit_exact():
             assert a.shape == b.shape and np.array_equal(a.view(np.uint64), b.view(np.uint64)), (i, j)
             hits += len(a) > 0
     assert hits > 30
+
+
+def test_edge_worlds_bit_exact():
+    """The edge-case worlds of the device tests (ragged scene sizes, fixed
+    bodies only, one body, several iterations, collisions off, other substep
+    counts), so that the oracle those tests check against is pinned on them."""
+    from test_gpu_edges import _ragged_world
+
+    _compare(_ragged_world(), 120, every=30)
+    w = scenes.batched_uniform(4, first_seed=800)
+    for kw in ({"iters": 4}, {"collisions": False}, {"substeps": 1, "dt": 1.0 / 30.0},
+               {"substeps": 7, "dt": 1.0 / 45.0}):
+        _compare(w, 60, every=20, **kw)
