@@ -354,10 +354,13 @@ int32_t rp_world_manifold_group(const rp_world* w);
  * identical. Returns 0 or -1. */
 int32_t rp_world_set_narrow_block_cap(rp_world* w, int32_t blocks);
 
-/* Dynamic LDS bytes one narrow-phase block may use on the world's device
- * (the device's limit per block: 160 KiB on gfx950). A lane-group schedule
- * of G lanes per pair fits where 64 / G pairs' slices of 2 x (the world's
- * largest hull's vertices) x 24 bytes do. */
+/* Dynamic LDS bytes one narrow-phase block may use in this world. A
+ * lane-group schedule of G lanes per pair fits where 64 / G pairs' slices of
+ * 2 x (the world's largest hull's vertices) x 24 bytes do. The limit is
+ * 64 KiB where the 16-lane schedule fits in it; else it is the device's limit
+ * per block (160 KiB on gfx950), and the world then takes lane groups only
+ * in the fused schedule (manifold group 0), besides the one-lane schedule
+ * (1, 1). */
 int32_t rp_world_lds_limit(const rp_world* w);
 
 /* Concurrent views: graph replay cuts the world into up to 4 scene ranges

@@ -504,15 +504,20 @@ static bool build_image(const HostWorld& hw, DeviceImage& im) {
     int max_nv = 1;
     for (const HostHull& h : hw.hulls) max_nv = std::max(max_nv, (int)h.v.size());
     d.stage_cap = 2 * max_nv;
-    // LDS a narrow-phase block may stage hulls in: the device's limit per
-    // block (160 KiB on gfx950), 64 KiB if the query fails
-    {
+    // LDS a narrow-phase block may stage hulls in: 64 KiB where a lane-group
+    // schedule fits in it (16 lanes per pair takes the least), else the
+    // device's limit per block (160 KiB on gfx950), for hulls that would
+    // otherwise have no lane-group schedule; such worlds run lane groups in
+    // the fused kernel only (large_lds_split)
+    d.lds_limit = 64 * 1024;
+    if ((size_t)(64 / 16) * d.stage_cap * sizeof(V3) > (size_t)d.lds_limit) {
         int dev = 0, lim = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || lim <= 0)
-            lim = 64 * 1024;
-        d.lds_limit = lim;
-        if (lim > 64 * 1024) RP_CHECK(allow_narrow_lds(lim));
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
+            lim > d.lds_limit) {
+            d.lds_limit = lim;
+            RP_CHECK(allow_narrow_lds(lim));
+        }
     }
     // size classes: with hulls above SMALL_HULL_NV vertices, the pairs of two
     // smaller hulls get their own list and LDS slices sized for them (if the
@@ -1487,8 +1492,19 @@ uint32_t rp_world_adjoint_amplification(rp_world* w, double* out) {
     return 0;
 }
 
+// Worlds whose lane-group slices need more than 64 KiB of LDS per block (the
+// larger limit applies only where no lane group fits in 64 KiB) run lane
+// groups in the fused GJK + EPA + manifold kernel only: the split kernels
+// were never run with such slices. The one-lane split schedule (1, 1) uses
+// no LDS and stays available.
+static bool large_lds_split(const DevWorld& d, int narrow_group, int manifold_group) {
+    if (d.lds_limit <= 64 * 1024) return false;
+    return (manifold_group != 0 && narrow_group > 1) || manifold_group > 1;
+}
+
 int32_t rp_world_set_narrow_group(rp_world* w, int32_t lanes_per_pair) {
     if (!narrow_group_supported(w->im.d, lanes_per_pair)) return -1;
+    if (large_lds_split(w->im.d, lanes_per_pair, w->im.d.manifold_group)) return -1;
     (void)hipStreamSynchronize(w->stream);
     int old_g = w->im.d.narrow_group, old_b = w->im.d.narrow_blocks;
     w->im.d.narrow_group = lanes_per_pair;
@@ -1506,6 +1522,7 @@ int32_t rp_world_narrow_group(const rp_world* w) { return w->im.d.narrow_group; 
 
 int32_t rp_world_set_manifold_group(rp_world* w, int32_t lanes_per_pair) {
     if (lanes_per_pair != 0 && !narrow_group_supported(w->im.d, lanes_per_pair)) return -1;
+    if (large_lds_split(w->im.d, w->im.d.narrow_group, lanes_per_pair)) return -1;
     (void)hipStreamSynchronize(w->stream);
     int old_g = w->im.d.manifold_group, old_b = w->im.d.manifold_blocks;
     w->im.d.manifold_group = lanes_per_pair;

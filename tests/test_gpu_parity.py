@@ -102,9 +102,9 @@ def test_batched_ico162_bit_exact():
 def test_642_vertex_hulls_on_lane_groups_bit_exact():
     """Icospheres of 642 vertices beside cubes: a 16-lane group stages two
     such hulls in 30.8 KiB, so four groups take 123 KiB of a block's LDS,
-    which gfx950 allows (160 KiB). The world runs its large-hull pairs on
-    the 16-lane schedule and its cube pairs on the 8-lane one, and matches
-    the oracle bit for bit."""
+    which gfx950 allows (160 KiB). The world runs its pairs on the fused
+    16-lane schedule (large-hull pairs and cube pairs in their own lists)
+    and matches the oracle bit for bit; split schedules are refused."""
     world = scenes.batched_uniform(2, first_seed=40, n_cubes=6, n_icos=3, ico_subdiv=3)
     g = SimWorld("rp", world)
     lim = g.lib.rp_world_lds_limit(g.h)
@@ -112,6 +112,10 @@ def test_642_vertex_hulls_on_lane_groups_bit_exact():
     if lim < 4 * 2 * 642 * 24:
         pytest.skip(f"the device gives a block {lim} bytes of LDS")
     assert g.lib.rp_world_narrow_group(g.h) > 1, "the large hulls should fit a lane-group schedule"
+    # slices above 64 KiB run in the fused kernel only
+    assert g.lib.rp_world_set_manifold_group(g.h, 1) != 0
+    assert g.lib.rp_world_set_manifold_group(g.h, 16) != 0
+    assert g.lib.rp_world_manifold_group(g.h) == 0
     g.close()
     _pair(world, frames=45, check_every=15)
 
