@@ -677,7 +677,12 @@ def main():
                         "solve": (oc["ops"]["solve"], oc["body_substeps"], "body-substep")}[dom]
             traffic = committed_traffic(dom, {"narrow": narrow_group, "manifold": manifold_group,
                                               "solve": solve_lanes})
-            roofline = {"bound": "fp64-valu", "kernel": names[dom],
+            # the compute roof ("mfma" in the record's vocabulary): MI355X's FP64
+            # peak, the same for its vector and matrix units; the kernel's FP64
+            # work is VALU (no matrix-shaped work in the step)
+            roofline = {"bound": "mfma", "bound_detail": "FP64 compute roof (78.6 TFLOP/s, vector = matrix "
+                        "peak); the kernel issues FP64 VALU, no MFMA; HBM under 8 % of 8 TB/s",
+                        "kernel": names[dom],
                         "achieved": k["tflops"], "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                         "frac": k["tflops"] / FP64_PEAK_TFLOPS if k["tflops"] else None,
                         "traffic": traffic["bytes_per_launch"] if traffic else None,
