@@ -605,9 +605,44 @@ __device__ __forceinline__ void group_mem_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-// Value of lane 0 of the caller's aligned group of G lanes.
-template <int G> __device__ __forceinline__ int group_bcast(int v) { return __shfl(v, 0, G); }
-template <int G> __device__ __forceinline__ double group_bcast(double v) { return __shfl(v, 0, G); }
+// Lane exchanges of a lane group by DPP moves (register to register, no LDS
+// round trip): the group's lanes all take part (control flow is uniform per
+// group), and a group of 4, 8 or 16 lanes never spans two 16-lane rows.
+enum : int { DPP_QUAD_BCAST0 = 0x00, DPP_QUAD_BCAST3 = 0xFF, DPP_ROW_SHR0 = 0x110 };
+// Value of lane 0 of the caller's aligned group of G lanes: each quad takes
+// its first lane's value, then the upper quad (and the upper half of a
+// 16-lane group) takes the mirrored lane's, which holds lane 0's by then.
+template <int G> __device__ __forceinline__ int group_bcast(int v) {
+    static_assert(G == 4 || G == 8 || G == 16, "lane groups of 4, 8 or 16");
+    int gl = (int)(__lane_id() & (G - 1));
+    v = dpp_int<DPP_QUAD_BCAST0>(v);
+    if constexpr (G >= 8) { int m = dpp_int<DPP_HALF_MIRROR>(v); v = (gl & 4) ? m : v; }
+    if constexpr (G >= 16) { int m = dpp_int<DPP_MIRROR>(v); v = (gl & 8) ? m : v; }
+    return v;
+}
+// Value of the last lane (G - 1) of the caller's group, the same way.
+template <int G> __device__ __forceinline__ int group_bcast_last(int v) {
+    static_assert(G == 4 || G == 8 || G == 16, "lane groups of 4, 8 or 16");
+    int gl = (int)(__lane_id() & (G - 1));
+    v = dpp_int<DPP_QUAD_BCAST3>(v);
+    if constexpr (G >= 8) { int m = dpp_int<DPP_HALF_MIRROR>(v); v = (gl & 4) ? v : m; }
+    if constexpr (G >= 16) { int m = dpp_int<DPP_MIRROR>(v); v = (gl & 8) ? v : m; }
+    return v;
+}
+template <int G> __device__ __forceinline__ double group_bcast(double v) {
+    long long b = __double_as_longlong(v);
+    int lo = group_bcast<G>((int)b), hi = group_bcast<G>((int)(b >> 32));
+    return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
+}
+// Minimum over the group's lanes (every lane gets it).
+template <int G> __device__ __forceinline__ int group_min(int v) {
+    static_assert(G == 4 || G == 8 || G == 16, "lane groups of 4, 8 or 16");
+    v = min(v, dpp_int<DPP_XOR1>(v));
+    v = min(v, dpp_int<DPP_XOR2>(v));
+    if constexpr (G >= 8) v = min(v, dpp_int<DPP_HALF_MIRROR>(v));
+    if constexpr (G >= 16) v = min(v, dpp_int<DPP_MIRROR>(v));
+    return v;
+}
 template <int G> __device__ __forceinline__ V3 group_bcast(V3 v) {
     return v3(group_bcast<G>(v.x), group_bcast<G>(v.y), group_bcast<G>(v.z));
 }
@@ -744,8 +779,7 @@ __device__ int epa_group(const View& h1, const View& h2, const Simplex& s, V3& n
             P.dist[nf + i] = fd;
             if (!ok && i < first_bad) first_bad = i;
         }
-#pragma unroll
-        for (int off = 1; off < G; off <<= 1) first_bad = min(first_bad, __shfl_xor(first_bad, off, G));
+        first_bad = group_min<G>(first_bad);
         if (first_bad < nnew) {
             if (leader) err |= ERR_EPA_DEGENERATE;
             return -1;
@@ -859,10 +893,19 @@ RP_HD int clip_pass(const V3* in, int n, const Plane& p, V3* out, bool remove, u
 // over the group, so the polygon equals clip_pass's point for point.
 template <int G> __device__ __forceinline__ int group_inclusive_sum(int v) {
     int gl = (int)(__lane_id() & (G - 1));
-#pragma unroll
-    for (int d = 1; d < G; d <<= 1) {
-        int o = __shfl_up(v, d, G);
-        v += gl >= d ? o : 0;
+    // row_shr:d reads lane - d of the same 16-lane row; every lane moves its
+    // value before the group's first d lanes drop what they received
+    int o = dpp_int<DPP_ROW_SHR0 + 1>(v);
+    v += gl >= 1 ? o : 0;
+    o = dpp_int<DPP_ROW_SHR0 + 2>(v);
+    v += gl >= 2 ? o : 0;
+    if constexpr (G >= 8) {
+        o = dpp_int<DPP_ROW_SHR0 + 4>(v);
+        v += gl >= 4 ? o : 0;
+    }
+    if constexpr (G >= 16) {
+        o = dpp_int<DPP_ROW_SHR0 + 8>(v);
+        v += gl >= 8 ? o : 0;
     }
     return v;
 }
@@ -894,7 +937,7 @@ __device__ int clip_pass_group(const V3* in, int n, const Plane& p, V3* out, boo
             }
         }
         int incl = group_inclusive_sum<G>(cnt);
-        int total = __shfl(incl, G - 1, G);
+        int total = group_bcast_last<G>(incl);
         int pos = m + incl - cnt;
         if (cnt >= 1 && pos < cap) out[pos] = o0;
         if (cnt == 2 && pos + 1 < cap) out[pos + 1] = o1;
@@ -1079,7 +1122,7 @@ __device__ int contacts_from_clip_group(const V3* pts, int n, const Plane& rp, V
         }
         int k = keep ? 1 : 0;
         int incl = group_inclusive_sum<G>(k);
-        int total = __shfl(incl, G - 1, G);
+        int total = group_bcast_last<G>(incl);
         int pos = nc + incl - k;
         if (keep && pos < MAXC) out[pos] = c;
         nc += total;
