@@ -35,6 +35,17 @@ __device__ __forceinline__ bool broad_hit(const Body& e1, const Body& e2) {
     return broad_hit(e1.x, e1.radius, e2.x, e2.radius);
 }
 
+// The frame's records of live pair g between global bodies b1 and b2: the
+// bodies and their hull instances, and both hulls' local-vertex offsets and
+// counts (so that the narrow phase loads the vertices right after the
+// records). A hull index < 0 (sphere or compound body) gives a zero hull.
+__device__ __forceinline__ void write_live_records(const DevWorld& w, int g, int b1, int b2) {
+    int h1 = w.body_hull[b1], h2 = w.body_hull[b2];
+    w.live_rec[g] = make_int4(b1, b2, h1, h2);
+    w.live_hull[g] = make_int4(h1 >= 0 ? w.hull_v_off[h1] : 0, h2 >= 0 ? w.hull_v_off[h2] : 0,
+                               h1 >= 0 ? w.hull_nv[h1] : 0, h2 >= 0 ? w.hull_nv[h2] : 0);
+}
+
 // Lock-free union-find over scene-local parent indices (islands,
 // broad.cpp:31-116): any order of the unions gives the same partition.
 __device__ __forceinline__ int uf_load(const int32_t* p, int x) {

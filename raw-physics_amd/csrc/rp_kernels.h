@@ -68,6 +68,7 @@ struct DevWorld {
     int32_t* live_prefix;           // [ns+1]
     int2* live_global;              // [pair capacity] (scene, pair slot)
     int4* live_rec;                 // [pair capacity] (body 1, body 2, hull instance 1, hull instance 2)
+    int4* live_hull;                // [pair capacity] (first local vertex, vertex count) of both hulls
     int32_t* ncontact;              // [pair capacity]
     V3* pair_normal;                // [pair capacity] EPA normal of the pairs queued for the manifold
     int32_t* mlist;                 // [pair capacity] manifold queue: live-pair slots

@@ -174,7 +174,7 @@ __global__ void k_scatter_live(DevWorld w) {
             int2 p = w.pairs[slot];
             b1 = b0 + p.x;
             b2 = b0 + p.y;
-            w.live_rec[g0 + m] = make_int4(b1, b2, w.body_hull[b1], w.body_hull[b2]);
+            write_live_records(w, g0 + m, b1, b2);
             w.need[b1] = 1;  // same value from every writer
             w.need[b2] = 1;
         }
@@ -323,30 +323,34 @@ __device__ __forceinline__ void pair_views(const DevWorld& w, int g, V3* st, int
     h1 = hull_view_of(w, b1, r.z, w.body_tv_off[b1]);
     h2 = hull_view_of(w, b2, r.w, w.body_tv_off[b2]);
     if constexpr (G > 1) {
-        const V3* lv1 = w.hull_v + w.hull_v_off[r.z];
-        const V3* lv2 = w.hull_v + w.hull_v_off[r.w];
+        // vertex offsets and counts from the pair's record: the vertex loads
+        // depend on that one load only
+        int4 hv = w.live_hull[g];
+        const V3* lv1 = w.hull_v + hv.x;
+        const V3* lv2 = w.hull_v + hv.y;
+        int nv1 = hv.z, nv2 = hv.w;
         const M4 M1 = w.model[b1], M2 = w.model[b2];
         // both hulls together, two vertices of each per lane and batch, all
         // loaded before any is transformed
-        int n = h1.nv > h2.nv ? h1.nv : h2.nv;
+        int n = nv1 > nv2 ? nv1 : nv2;
         for (int i = gl; i < n; i += 2 * G) {
             V3 va[2], vb[2];
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
                 int j = i + k * G;
-                if (j < h1.nv) va[k] = lv1[j];
-                if (j < h2.nv) vb[k] = lv2[j];
+                if (j < nv1) va[k] = lv1[j];
+                if (j < nv2) vb[k] = lv2[j];
             }
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
                 int j = i + k * G;
-                if (j < h1.nv) st[j] = transform_vertex(M1, va[k]);
-                if (j < h2.nv) st[h1.nv + j] = transform_vertex(M2, vb[k]);
+                if (j < nv1) st[j] = transform_vertex(M1, va[k]);
+                if (j < nv2) st[nv1 + j] = transform_vertex(M2, vb[k]);
             }
         }
         group_lds_sync();
         h1.tv = st;
-        h2.tv = st + h1.nv;
+        h2.tv = st + nv1;
     }
 }
 

@@ -380,7 +380,7 @@ __global__ void k_big_scatter(DevWorld w, BigScene b) {
             int2 p = w.pairs[slot];
             b1 = b.b0 + p.x;
             b2 = b.b0 + p.y;
-            w.live_rec[g0 + m] = make_int4(b1, b2, w.body_hull[b1], w.body_hull[b2]);
+            write_live_records(w, g0 + m, b1, b2);
             w.need[b1] = 1;  // same value from every writer
             w.need[b2] = 1;
         }

@@ -586,6 +586,7 @@ static bool build_image(const HostWorld& hw, DeviceImage& im) {
     ok &= im.alloc(d.live_prefix, d.ns + 1 + kMaxViews);  // a view's prefix ends one slot past its scenes
     ok &= im.alloc(d.live_global, (size_t)pcap);
     ok &= im.alloc(d.live_rec, (size_t)pcap);
+    ok &= im.alloc(d.live_hull, (size_t)pcap);
     ok &= im.alloc(d.ncontact, (size_t)pcap);
     ok &= im.alloc(d.pair_normal, (size_t)pcap);
     ok &= im.alloc(d.mlist, (size_t)pcap);
@@ -879,7 +880,7 @@ static bool build_views(rp_world* w) {
         v.pair_count += s0; v.live_count += s0; v.live_prefix += s0 + k;
         v.scene_nlevels += s0; v.scene_big += s0; v.scene_err += s0;
         v.item_lstart += s0; v.item_cursor += s0;
-        v.live_global += P0; v.live_rec += P0; v.ncontact += P0; v.pair_normal += P0; v.mlist += P0; v.pair_lists += P0;
+        v.live_global += P0; v.live_rec += P0; v.live_hull += P0; v.ncontact += P0; v.pair_normal += P0; v.mlist += P0; v.pair_lists += P0;
         v.contacts += P0 * MAXC;
         v.counters += CNT_COUNT * k;
         v.xf += im.body_xf_off[b0];
